@@ -1,0 +1,208 @@
+/*
+ * lpm_mi355x.h — C ABI of the MI355X (gfx950) direct-sum library.
+ *
+ * This is the drop-in boundary for LPM's all-pairs spherical direct sums and
+ * the RK4 steppers that drive them.  Every entry point takes plain pointers
+ * and sizes; no C++, Kokkos, HIP or torch types cross it.  Each declaration
+ * names the reference interface it replaces (pbosler/lpm file:line).
+ *
+ * Conventions
+ *  - Device pointers: every array argument is device memory unless the
+ *    parameter name ends in _host.
+ *  - Coordinates / vectors: element (i, k), k in {0,1,2}, lives at
+ *    base[i * es + k * cs].  A Kokkos View<double*[3]> maps without copies:
+ *    LayoutLeft (GPU backends) es = 1, cs = view.stride(1);
+ *    LayoutRight (OpenMP) es = 3, cs = 1.
+ *  - Masks are one byte per face (Kokkos View<bool*>): nonzero = divided face,
+ *    excluded as a source.  A NULL mask means no face is divided.
+ *  - Streams are hipStream_t passed as void* (NULL = the default stream).
+ *    Calls are stream-ordered and asynchronous unless stated otherwise.
+ *  - Return value: LPM_OK (0) or a negative lpm_status; the message is
+ *    available from lpm_last_error(ctx).  Nothing throws across the ABI.
+ *  - A context is single-threaded (not reentrant); distinct contexts may be
+ *    used from distinct host threads on distinct streams.
+ */
+#ifndef LPM_MI355X_H
+#define LPM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LPM_ABI_VERSION 1
+
+typedef enum lpm_status {
+  LPM_OK = 0,
+  LPM_ERR_ARG = -1,       /* invalid argument (null pointer, size, stride) */
+  LPM_ERR_HIP = -2,       /* HIP runtime error */
+  LPM_ERR_NONFINITE = -3, /* LPM_FLAG_CHECK_FINITE found a NaN/Inf output */
+  LPM_ERR_STATE = -4,     /* call out of order / incompatible context state */
+  LPM_ERR_NOMEM = -5,     /* device allocation failed */
+  LPM_ERR_INTERNAL = -6
+} lpm_status;
+
+typedef enum lpm_precision {
+  LPM_FP64 = 0,  /* FP64 pair arithmetic, the reference's formula order */
+  LPM_MIXED = 1  /* FP32 pair kernel (|x-y|^2/2 form), FP64 carries/state */
+} lpm_precision;
+
+/* Flags (bitwise or). */
+#define LPM_FLAG_CHECK_FINITE 0x1u   /* scan outputs, return LPM_ERR_NONFINITE */
+#define LPM_FLAG_SKIP_MASKED_TARGETS 0x2u /* collocated: leave divided-face
+                                            targets untouched (not computed) */
+#define LPM_RK4_REFERENCE_COMPAT 0x10u /* reproduce the reference's stage
+                                          arithmetic (see lpm_bve_rk4_step) */
+#define LPM_RK4_RENORMALIZE 0x20u /* project stage and final positions onto
+                                     the unit sphere (FP64) */
+#define LPM_RK4_USE_GRAPH 0x40u   /* capture the step in a hipGraph once and
+                                     replay it on later identical calls */
+
+typedef struct lpm_ctx lpm_ctx;
+
+/* ------------------------------------------------------------ context ---- */
+int lpm_abi_version(void);
+
+/* Creates a context bound to HIP device `device` (the device is made current
+ * on the calling thread).  No reference counterpart: replaces the implicit
+ * Kokkos::DefaultExecutionSpace (src/lpm_kokkos_defs.hpp:18). */
+int lpm_ctx_create(int device, lpm_ctx** out);
+int lpm_ctx_destroy(lpm_ctx* ctx);
+
+/* Last error message of `ctx` (or of the calling thread when ctx is NULL,
+ * e.g. after a failed lpm_ctx_create).  Never NULL. */
+const char* lpm_last_error(const lpm_ctx* ctx);
+
+/* Blocks until all work the context issued on `stream` has finished. */
+int lpm_stream_synchronize(lpm_ctx* ctx, void* stream);
+
+/* -------------------------------------------------------------- mesh ---- */
+/* Uniformly refined spherical quadtree mesh (host memory).  Replaces
+ * MeshSeed<Seed> + PolyMesh2d<Seed>::tree_init
+ * (src/mesh/lpm_mesh_seed.cpp:20-304, src/mesh/lpm_polymesh2d_impl.hpp:25-42)
+ * as the producer of the direct sum's inputs; particles are created in the
+ * reference's order. */
+typedef struct lpm_mesh lpm_mesh;
+#define LPM_SEED_ICOS_TRI 0   /* IcosTriSphereSeed */
+#define LPM_SEED_CUBED_SPHERE 1 /* CubedSphereSeed */
+
+int lpm_mesh_create(int seed, int depth, lpm_mesh** out);
+int lpm_mesh_destroy(lpm_mesh* mesh);
+/* Counts: vertices, stored faces (leaves + divided), leaf faces, edges. */
+int lpm_mesh_sizes(const lpm_mesh* mesh, int64_t* n_verts, int64_t* n_faces,
+                   int64_t* n_leaves, int64_t* n_edges);
+/* MeshSeed::set_max_allocations (src/mesh/lpm_mesh_seed.cpp:266-279). */
+int lpm_mesh_max_allocations(int seed, int depth, int64_t* n_verts,
+                             int64_t* n_edges, int64_t* n_faces);
+/* Copies into host arrays; xyz arrays are SoA [3][n] (x row, y row, z row).
+ * Any output may be NULL. */
+int lpm_mesh_copy_vertices(const lpm_mesh* mesh, double* xyz_host);
+int lpm_mesh_copy_faces(const lpm_mesh* mesh, double* xyz_host,
+                        double* area_host, uint8_t* mask_host,
+                        int32_t* parent_host);
+/* Face corner vertex indices, [n_faces][verts_per_face] row-major. */
+int lpm_mesh_copy_face_vertices(const lpm_mesh* mesh, int32_t* verts_host,
+                                int* verts_per_face);
+
+/* --------------------------------------------------------- BVE sums ---- */
+/* Velocity of the barotropic vorticity equation on the unit sphere by the
+ * all-pairs Biot-Savart sum over the undivided faces:
+ *   u(x_i) = sum_j (x_i cross y_j) (-zeta_j A_j) / (4 pi (1 - x_i.y_j + eps^2))
+ * Replaces BVEVertexVelocity (collocated = 0) and BVEFaceVelocity
+ * (collocated = 1) with VelocityReduceDistinct / VelocityReduceCollocated and
+ * biot_savart (src/lpm_bve_sphere_kernels.hpp:53-82, 179-211, 249-274,
+ * 365-394; src/lpm_sphere_functions.hpp:44-57).
+ *   tgt_*: target positions; n_tgt targets.  With collocated = 1 the targets
+ *          are the sources (pass the same array, n_tgt = n_src) and j == i
+ *          is skipped; with LPM_FLAG_SKIP_MASKED_TARGETS, divided-face
+ *          targets are not computed and their output is left untouched.
+ *   src_*: face centres, relative vorticity, area, mask; n_src stored faces
+ *          (the reference's n_faces_host(), divided faces included).
+ *   eps:   kernel smoothing (the reference uses 0).
+ *   out_u: velocity at the targets (may alias nothing else). */
+int lpm_bve_velocity(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                     int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                     int64_t src_es, int64_t src_cs, const double* src_zeta,
+                     const double* src_area, const uint8_t* src_mask,
+                     int64_t n_src, int collocated, double eps, int precision,
+                     unsigned flags, double* out_u, int64_t out_es,
+                     int64_t out_cs, void* stream);
+
+/* Stream function psi(x_i) = sum_j log(1 - x_i.y_j) (-zeta_j A_j) / (4 pi).
+ * Replaces BVEVertexStreamFn / BVEFaceStreamFn with StreamReduceDistinct /
+ * StreamReduceCollocated and greens_fn (src/lpm_bve_sphere_kernels.hpp:20-48,
+ * 141-170, 218-242, 329-356; src/lpm_sphere_functions.hpp:21-29).
+ * Arguments as lpm_bve_velocity; eps must be 0. */
+int lpm_bve_stream_fn(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                      int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                      int64_t src_es, int64_t src_cs, const double* src_zeta,
+                      const double* src_area, const uint8_t* src_mask,
+                      int64_t n_src, int collocated, int precision,
+                      unsigned flags, double* out_psi, void* stream);
+
+/* ---------------------------------------------------------- BVE RK4 ---- */
+/* Fields of BVESphere<Seed> the stepper reads and writes (device memory).
+ * Positions/velocities use (es, cs) strides as above. */
+typedef struct lpm_bve_fields {
+  int64_t n_verts;       /* passive particles (mesh vertices) */
+  double* vert_x;        /* in/out positions */
+  int64_t vert_x_es, vert_x_cs;
+  double* vert_zeta;     /* in/out relative vorticity */
+  double* vert_u;        /* in: velocity at vert_x; out: at the new vert_x */
+  int64_t vert_u_es, vert_u_cs;
+  int64_t n_faces;       /* active particles: stored faces incl. divided */
+  double* face_x;
+  int64_t face_x_es, face_x_cs;
+  double* face_zeta;
+  double* face_u;
+  int64_t face_u_es, face_u_cs;
+  const double* face_area;
+  const uint8_t* face_mask; /* NULL: no divided faces */
+} lpm_bve_fields;
+
+/* Sets vert_u and face_u from the current positions and vorticity
+ * (BVESphere::init_velocity, src/lpm_bve_sphere_impl.hpp:181-205). */
+int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
+                          int precision, unsigned flags, void* stream);
+
+/* One classical RK4 step of (x, zeta) for vertices and faces, with
+ * dzeta/dt = -2 Omega u_z and 4 direct-sum evaluations (BVERK4::
+ * advance_timestep, src/lpm_bve_rk4_impl.hpp:55-167; BVERK4Update :12-53;
+ * BVEVorticityTendency src/lpm_bve_sphere_kernels.hpp:399-413).
+ * On entry vert_u/face_u must hold the velocity at the current positions
+ * (from lpm_bve_init_velocity or the previous step); on exit they hold the
+ * velocity at the new positions.
+ * LPM_RK4_REFERENCE_COMPAT: the face vorticity update uses (k1, k2, k4, k4)
+ * as the reference does (src/lpm_bve_rk4_impl.hpp:157); without it,
+ * (k1, k2, k3, k4).  LPM_RK4_RENORMALIZE: stage and final positions are
+ * projected onto the unit sphere.  LPM_FLAG_SKIP_MASKED_TARGETS: divided
+ * faces are neither evaluated nor advanced.  LPM_RK4_USE_GRAPH: the step's
+ * launches are captured into a hipGraph on first use and replayed.
+ * Stage buffers are owned by the context (sized on first use). */
+int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
+                     double Omega, int precision, unsigned flags,
+                     void* stream);
+
+/* ------------------------------------------------------------ timing ---- */
+/* HIP events recorded around every direct-sum launch, on the stream the
+ * kernel is launched on (disabled inside graph capture).  Counterpart of the
+ * reference's Timer with Kokkos::fence (src/util/lpm_timer.hpp:16-51).
+ * lpm_timing_enable resets all records; lpm_timing_query synchronises on the
+ * pending events and returns the accumulated kernel time, launch count and
+ * pair count (targets x stored sources) of one kernel family. */
+#define LPM_KERNEL_BVE_VEL_F64 0
+#define LPM_KERNEL_BVE_VEL_MIXED 1
+#define LPM_KERNEL_BVE_PSI 2
+#define LPM_KERNEL_SWE_F64 3
+#define LPM_KERNEL_SWE_MIXED 4
+#define LPM_KERNEL_BVE_BATCHED 5
+int lpm_timing_enable(lpm_ctx* ctx, int enable);
+int lpm_timing_query(lpm_ctx* ctx, int kernel_id, double* total_ms,
+                     int64_t* launches, double* pairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPM_MI355X_H */

@@ -1,0 +1,394 @@
+// BVE RK4 stepper on the device: fused O(N) stage kernels around the direct
+// sums, following BVERK4::advance_timestep (src/lpm_bve_rk4_impl.hpp:55-167)
+// stage by stage.  The reference's per-stage KokkosBlas::scal/update calls
+// and BVEVorticityTendency launches (30 O(N) launches per step) become one
+// stage kernel per particle kind and stage.  Optionally the whole step is
+// captured into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+
+#include "internal.hpp"
+
+struct lpm_ctx::GraphEntry {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t epoch = 0;
+  ~GraphEntry() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace lpm_amd {
+
+namespace {
+
+// Stage storage of one particle kind, SoA with leading dimension n:
+//   k[s]  : dt*u of stage s (3n), s = 0..3
+//   zk[s] : vorticity increment of stage s (n)
+//   xw    : work positions (3n), zw : work vorticity (n)
+struct StageBufs {
+  double* k[4];
+  double* zk[4];
+  double* xw;
+  double* zw;
+  int64_t n;
+};
+
+StageBufs carve(DevBuf& buf, int64_t n) {
+  const int64_t nn = std::max<int64_t>(n, 1);
+  double* p = static_cast<double*>(buf.ensure(sizeof(double) * 20 * nn));
+  StageBufs b;
+  for (int s = 0; s < 4; ++s) {
+    b.k[s] = p + (4 * s) * nn;
+    b.zk[s] = p + (4 * s + 3) * nn;
+  }
+  b.xw = p + 16 * nn;
+  b.zw = p + 19 * nn;
+  b.n = n;
+  return b;
+}
+
+struct StageArgs {
+  const double* x;  // state positions (strided)
+  int64_t xes, xcs;
+  const double* zeta;
+  const double* u;  // velocity just evaluated (strided)
+  int64_t ues, ucs;
+  const uint8_t* skip;
+  double* k;   // this stage's dt*u (SoA, ld = n)
+  double* zk;  // this stage's vorticity increment
+  double* xw;  // next work positions (SoA), nullptr after stage 4
+  double* zw;
+  int64_t n;
+  double dt, Omega, a;
+  int renorm;
+};
+
+// x_k = dt u; zeta_k = -2 Omega u_z dt (BVEVorticityTendency);
+// xwork = x + a x_k, zetawork = zeta + a zeta_k (KokkosBlas::update).
+__global__ __launch_bounds__(256) void rk_stage_kernel(StageArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.skip && A.skip[i]) return;
+  const double* u = A.u + i * A.ues;
+  const double u0 = u[0], u1 = u[A.ucs], u2 = u[2 * A.ucs];
+  const double k0 = A.dt * u0, k1 = A.dt * u1, k2 = A.dt * u2;
+  const double zk = -2.0 * A.Omega * u2 * A.dt;
+  const int64_t n = A.n;
+  A.k[i] = k0;
+  A.k[i + n] = k1;
+  A.k[i + 2 * n] = k2;
+  A.zk[i] = zk;
+  if (A.xw) {
+    const double* x = A.x + i * A.xes;
+    double w0 = x[0] + A.a * k0;
+    double w1 = x[A.xcs] + A.a * k1;
+    double w2 = x[2 * A.xcs] + A.a * k2;
+    if (A.renorm) {
+      const double r = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+      w0 *= r;
+      w1 *= r;
+      w2 *= r;
+    }
+    A.xw[i] = w0;
+    A.xw[i + n] = w1;
+    A.xw[i + 2 * n] = w2;
+    A.zw[i] = A.zeta[i] + A.a * zk;
+  }
+}
+
+struct UpdateArgs {
+  double* x;
+  int64_t xes, xcs;
+  double* zeta;
+  const uint8_t* skip;
+  const double* k[4];
+  const double* zk[4];  // already permuted for compat mode
+  int64_t n;
+  int renorm;
+};
+
+// BVERK4Update: x += (k1 + k4)/6 + (k2 + k3)/3, same for zeta.
+__global__ __launch_bounds__(256) void rk_update_kernel(UpdateArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.skip && A.skip[i]) return;
+  constexpr double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
+  const int64_t n = A.n;
+  double* x = A.x + i * A.xes;
+  double v[3];
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = i + c * n;
+    v[c] = x[c * A.xcs] + (sixth * (A.k[0][o] + A.k[3][o]) +
+                           third * (A.k[1][o] + A.k[2][o]));
+  }
+  if (A.renorm) {
+    const double r = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] *= r;
+    v[1] *= r;
+    v[2] *= r;
+  }
+  for (int c = 0; c < 3; ++c) x[c * A.xcs] = v[c];
+  A.zeta[i] += sixth * (A.zk[0][i] + A.zk[3][i]) +
+               third * (A.zk[1][i] + A.zk[2][i]);
+}
+
+inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
+
+void velocity_eval(lpm_ctx* ctx, const lpm_bve_fields& f, const double* vx,
+                   int64_t vxes, int64_t vxcs, const double* fx, int64_t fxes,
+                   int64_t fxcs, const double* fzeta, const RankMap& map,
+                   int precision, unsigned flags, hipStream_t s) {
+  PackedSources ps = pack_sources(ctx, CrdIn{fx, fxes, fxcs}, fzeta,
+                                  f.face_area, map, precision, s);
+  TargetSeg segs[2];
+  segs[0].x = CrdIn{vx, vxes, vxcs};
+  segs[0].n = f.n_verts;
+  segs[0].out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
+  segs[1] = face_targets(CrdIn{fx, fxes, fxcs}, map, f.face_mask,
+                         (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
+                         CrdOut{f.face_u, f.face_u_es, f.face_u_cs}, nullptr);
+  launch_bve_velocity(ctx, ps, segs, 2, 0.0, precision, s);
+}
+
+void validate(const lpm_bve_fields* f) {
+  LPM_ARG_CHECK(f, "null fields");
+  LPM_ARG_CHECK(f->n_verts >= 0 && f->n_faces >= 0, "negative sizes");
+  if (f->n_verts > 0) {
+    LPM_ARG_CHECK(f->vert_x && f->vert_zeta && f->vert_u,
+                  "vertex fields are NULL");
+    LPM_ARG_CHECK(f->vert_x_es > 0 && f->vert_x_cs > 0 && f->vert_u_es > 0 &&
+                      f->vert_u_cs > 0,
+                  "vertex strides must be > 0");
+  }
+  LPM_ARG_CHECK(f->n_faces > 0, "at least one face is required");
+  LPM_ARG_CHECK(f->face_x && f->face_zeta && f->face_u && f->face_area,
+                "face fields are NULL");
+  LPM_ARG_CHECK(f->face_x_es > 0 && f->face_x_cs > 0 && f->face_u_es > 0 &&
+                    f->face_u_cs > 0,
+                "face strides must be > 0");
+}
+
+void issue_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt,
+                double Omega, int precision, unsigned flags, StageBufs& V,
+                StageBufs& F, hipStream_t s) {
+  const int renorm = (flags & LPM_RK4_RENORMALIZE) ? 1 : 0;
+  const uint8_t* fskip =
+      (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
+  const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+  const double coef[3] = {0.5, 0.5, 1.0};
+
+  for (int stage = 0; stage < 4; ++stage) {
+    if (stage > 0)
+      velocity_eval(ctx, f, V.xw, 1, V.n, F.xw, 1, F.n, F.zw, map, precision,
+                    flags, s);
+    StageArgs a{};
+    a.dt = dt;
+    a.Omega = Omega;
+    a.a = stage < 3 ? coef[stage] : 0.0;
+    a.renorm = renorm;
+    if (f.n_verts > 0) {
+      a.x = f.vert_x;
+      a.xes = f.vert_x_es;
+      a.xcs = f.vert_x_cs;
+      a.zeta = f.vert_zeta;
+      a.u = f.vert_u;
+      a.ues = f.vert_u_es;
+      a.ucs = f.vert_u_cs;
+      a.skip = nullptr;
+      a.k = V.k[stage];
+      a.zk = V.zk[stage];
+      a.xw = stage < 3 ? V.xw : nullptr;
+      a.zw = V.zw;
+      a.n = f.n_verts;
+      rk_stage_kernel<<<nblk(a.n), 256, 0, s>>>(a);
+      LPM_HIP_CHECK(hipGetLastError());
+    }
+    a.x = f.face_x;
+    a.xes = f.face_x_es;
+    a.xcs = f.face_x_cs;
+    a.zeta = f.face_zeta;
+    a.u = f.face_u;
+    a.ues = f.face_u_es;
+    a.ucs = f.face_u_cs;
+    a.skip = fskip;
+    a.k = F.k[stage];
+    a.zk = F.zk[stage];
+    a.xw = stage < 3 ? F.xw : nullptr;
+    a.zw = F.zw;
+    a.n = f.n_faces;
+    rk_stage_kernel<<<nblk(a.n), 256, 0, s>>>(a);
+    LPM_HIP_CHECK(hipGetLastError());
+  }
+
+  UpdateArgs u{};
+  u.renorm = renorm;
+  if (f.n_verts > 0) {
+    u.x = f.vert_x;
+    u.xes = f.vert_x_es;
+    u.xcs = f.vert_x_cs;
+    u.zeta = f.vert_zeta;
+    u.skip = nullptr;
+    for (int k = 0; k < 4; ++k) {
+      u.k[k] = V.k[k];
+      u.zk[k] = V.zk[k];
+    }
+    u.n = f.n_verts;
+    rk_update_kernel<<<nblk(u.n), 256, 0, s>>>(u);
+    LPM_HIP_CHECK(hipGetLastError());
+  }
+  u.x = f.face_x;
+  u.xes = f.face_x_es;
+  u.xcs = f.face_x_cs;
+  u.zeta = f.face_zeta;
+  u.skip = fskip;
+  for (int k = 0; k < 4; ++k) {
+    u.k[k] = F.k[k];
+    u.zk[k] = F.zk[k];
+  }
+  // reference face update passes (k1, k2, k4, k4) for the vorticity
+  if (flags & LPM_RK4_REFERENCE_COMPAT) u.zk[2] = F.zk[3];
+  u.n = f.n_faces;
+  rk_update_kernel<<<nblk(u.n), 256, 0, s>>>(u);
+  LPM_HIP_CHECK(hipGetLastError());
+
+  velocity_eval(ctx, f, f.vert_x, f.vert_x_es, f.vert_x_cs, f.face_x,
+                f.face_x_es, f.face_x_cs, f.face_zeta, map, precision, flags, s);
+}
+
+std::string graph_key(const lpm_bve_fields& f, double dt, double Omega,
+                      int precision, unsigned flags, hipStream_t s) {
+  std::ostringstream k;
+  k << "bve|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
+    << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_u << '|'
+    << f.vert_u_es << '|' << f.vert_u_cs << '|' << f.n_faces << '|'
+    << f.face_x << '|' << f.face_x_es << '|' << f.face_x_cs << '|'
+    << f.face_zeta << '|' << f.face_u << '|' << f.face_u_es << '|'
+    << f.face_u_cs << '|' << f.face_area << '|'
+    << static_cast<const void*>(f.face_mask) << '|';
+  k.precision(17);
+  k << dt << '|' << Omega << '|' << precision << '|' << flags << '|' << s;
+  return k.str();
+}
+
+}  // namespace
+
+}  // namespace lpm_amd
+
+using namespace lpm_amd;
+
+extern "C" {
+
+int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
+                          int precision, unsigned flags, void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
+                  f->face_x_es, f->face_x_cs, f->face_zeta, map, precision,
+                  flags, s);
+    if (flags & LPM_FLAG_CHECK_FINITE) {
+      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
+      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
+    }
+    return LPM_OK;
+  } catch (const Error& e) {
+    set_error(ctx, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(ctx, e.what());
+    return LPM_ERR_INTERNAL;
+  }
+}
+
+int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
+                     double Omega, int precision, unsigned flags,
+                     void* stream) {
+  bool capturing = false;
+  hipStream_t s = as_stream(stream);
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
+                  "dt and Omega must be finite");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    StageBufs V = carve(ctx->rk_v, f->n_verts);
+    StageBufs F = carve(ctx->rk_f, f->n_faces);
+    if (flags & LPM_RK4_USE_GRAPH) {
+      LPM_ARG_CHECK(s != nullptr,
+                    "LPM_RK4_USE_GRAPH needs a non-default stream");
+      const std::string key = graph_key(*f, dt, Omega, precision, flags, s);
+      auto it = ctx->graphs.find(key);
+      if (it != ctx->graphs.end() &&
+          it->second->epoch != g_workspace_epoch.load()) {
+        ctx->graphs.erase(it);
+        it = ctx->graphs.end();
+      }
+      if (it == ctx->graphs.end()) {
+        // make sure every workspace exists before capture (no allocation
+        // may happen while the stream is being captured)
+        {
+          const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+          pack_sources(ctx, CrdIn{f->face_x, f->face_x_es, f->face_x_cs},
+                       f->face_zeta, f->face_area, map, precision, s);
+          LPM_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        auto entry = std::make_shared<lpm_ctx::GraphEntry>();
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        LPM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        try {
+          issue_step(ctx, *f, dt, Omega, precision, flags, V, F, s);
+        } catch (...) {
+          ctx->timing = timing;
+          throw;
+        }
+        ctx->timing = timing;
+        capturing = false;
+        LPM_HIP_CHECK(hipStreamEndCapture(s, &entry->graph));
+        LPM_HIP_CHECK(
+            hipGraphInstantiate(&entry->exec, entry->graph, nullptr, nullptr, 0));
+        entry->epoch = g_workspace_epoch.load();
+        it = ctx->graphs.emplace(key, entry).first;
+      }
+      LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
+    } else {
+      issue_step(ctx, *f, dt, Omega, precision, flags, V, F, s);
+    }
+    if (flags & LPM_FLAG_CHECK_FINITE) {
+      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
+      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
+    }
+    return LPM_OK;
+  } catch (const Error& e) {
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+    set_error(ctx, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+    set_error(ctx, e.what());
+    return LPM_ERR_INTERNAL;
+  }
+}
+
+}  // extern "C"

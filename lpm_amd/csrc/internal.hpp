@@ -1,0 +1,192 @@
+// Internal declarations shared by the HIP translation units of
+// liblpm_mi355x.so.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lpm_mi355x.h"
+
+namespace lpm_amd {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define LPM_HIP_CHECK(expr)                                                 \
+  do {                                                                      \
+    hipError_t lpm_e_ = (expr);                                             \
+    if (lpm_e_ != hipSuccess)                                               \
+      throw ::lpm_amd::Error(LPM_ERR_HIP, std::string(#expr) + " failed: " + \
+                                              hipGetErrorString(lpm_e_));   \
+  } while (0)
+
+#define LPM_ARG_CHECK(cond, msg)                                  \
+  do {                                                            \
+    if (!(cond)) throw ::lpm_amd::Error(LPM_ERR_ARG, (msg));      \
+  } while (0)
+
+// Incremented whenever any workspace is (re)allocated; captured graphs
+// record it and are re-captured when it changed, so that a replay never
+// touches freed workspace memory.
+extern std::atomic<uint64_t> g_workspace_epoch;
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void* ensure(size_t nbytes);  // reallocates (contents lost) when too small
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// Strided coordinate/vector arrays: element (i, k) at base[i*es + k*cs].
+struct CrdIn {
+  const double* base;
+  int64_t es, cs;
+};
+struct CrdOut {
+  double* base;
+  int64_t es, cs;
+};
+
+// Compaction of a face mask: rank[i] = packed index of face i (-1 if
+// divided), inv[r] = face of packed index r, *d_count = number packed.
+// With no mask, rank and inv are the identity and not materialised.
+struct RankMap {
+  const int32_t* rank = nullptr;
+  const int32_t* inv = nullptr;
+  const int32_t* d_count = nullptr;
+  int64_t n = 0;
+};
+
+// A target set of one direct-sum launch.  Thread slot c (0 <= c < n, or
+// < *d_n when d_n is set) evaluates target i = index ? index[c] : c.
+struct TargetSeg {
+  CrdIn x{nullptr, 0, 0};
+  int64_t n = 0;
+  const int32_t* index = nullptr;      // slot -> target (compacted targets)
+  const int32_t* d_n = nullptr;        // device count of live slots
+  const int32_t* self_rank = nullptr;  // collocated: packed index of target
+                                       // i's own source, -1 if none
+  bool self_identity = false;          // collocated: self = slot c
+  const uint8_t* skip = nullptr;       // targets to leave untouched
+  CrdOut out{nullptr, 0, 0};           // velocity output
+  double* out_scalar = nullptr;        // stream-function output
+};
+
+// Sources after the prologue: undivided faces, compacted and packed.
+struct PackedSources {
+  const void* data = nullptr;      // double4 {x,y,z,s} (FP64) or float4
+                                   // {x',y',z',2s} tile-relative (mixed)
+  const double4* centers = nullptr; // mixed: tile centres (x, y, z, -)
+  const int32_t* count = nullptr;  // device: number of packed sources
+  int64_t capacity = 0;            // host upper bound (n stored faces)
+};
+
+// Kernel timing records (HIP events around direct-sum launches).
+enum KernelId : int {
+  KID_BVE_VEL_F64 = 0,
+  KID_BVE_VEL_MIXED = 1,
+  KID_BVE_PSI = 2,
+  KID_SWE_F64 = 3,
+  KID_SWE_MIXED = 4,
+  KID_BVE_BATCHED = 5,
+  KID_COUNT = 6
+};
+
+struct TimingRec {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0;
+  int64_t launches = 0;
+  double pairs = 0;
+};
+
+// Launch configuration of the tiled direct-sum kernels.
+struct VelConfig {
+  int variant = 0;  // index into the compiled kernel table
+};
+
+}  // namespace lpm_amd
+
+struct lpm_ctx {
+  int device = 0;
+  std::string last_error;
+  int num_cus = 0;
+  std::string arch;
+
+  // prologue workspaces
+  lpm_amd::DevBuf rank;         // int32 per stored face
+  lpm_amd::DevBuf inv;          // int32 per stored face (packed -> face)
+  lpm_amd::DevBuf block_counts; // int32 per scan block (+1 total)
+  lpm_amd::DevBuf packed64;     // double4 per source
+  lpm_amd::DevBuf packed32;     // float4 per source (mixed)
+  lpm_amd::DevBuf centers;      // double4 per tile (mixed)
+  lpm_amd::DevBuf flag;         // int32 non-finite flag
+  lpm_amd::DevBuf count_ident;  // int32 count when no mask
+
+  // RK4 stage buffers
+  lpm_amd::DevBuf rk_v, rk_f;   // vertex / face stage storage
+
+  // timing
+  bool timing = false;
+  lpm_amd::TimingRec timing_rec[lpm_amd::KID_COUNT];
+  std::vector<hipEvent_t> event_pool;
+
+  // graph cache for RK4 steps
+  struct GraphEntry;
+  std::map<std::string, std::shared_ptr<GraphEntry>> graphs;
+
+  // autotuned kernel variants, keyed by (kernel, size bucket)
+  std::map<std::string, int> tuned;
+};
+
+namespace lpm_amd {
+
+void set_error(lpm_ctx* ctx, const std::string& msg);
+hipStream_t as_stream(void* s);
+
+// Event bracket for a timed launch (no-op when timing is off).
+struct TimedLaunch {
+  lpm_ctx* ctx;
+  int kid;
+  hipStream_t stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double pairs;
+  TimedLaunch(lpm_ctx* c, int k, hipStream_t s, double p);
+  ~TimedLaunch() noexcept(false);
+};
+
+// ---- prologue (bve_sums.hip) ----
+RankMap build_rank(lpm_ctx* ctx, const uint8_t* mask, int64_t n,
+                   hipStream_t s);
+PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
+                           const double* area, const RankMap& map,
+                           int precision, hipStream_t s);
+// Face targets of a collocated sum: every stored face (self skipped by
+// rank) or, with skip_masked, only the undivided ones (compacted slots).
+TargetSeg face_targets(CrdIn x, const RankMap& map, const uint8_t* mask,
+                       bool skip_masked, CrdOut out, double* out_scalar);
+void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
+                         TargetSeg* segs, int nseg, double eps2, int precision,
+                         hipStream_t s);
+void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
+                          TargetSeg* segs, int nseg, hipStream_t s);
+void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
+                  int64_t cs, int ncomp, hipStream_t s);
+
+}  // namespace lpm_amd

@@ -1,0 +1,289 @@
+"""GPU parity of the BVE direct sums and RK4 stepper (HIP, through the C ABI)
+against the FP64 oracle on the same inputs.
+
+Tolerances: FP64 kernels <= 1e-12 relative (L2 and Linf) against the oracle
+-- the order of summation differs (tiles, factored cross product, Newton
+reciprocal) but the arithmetic is FP64 throughout; the mixed-precision
+kernel <= 1e-6 relative L2 against the FP64 kernel.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import aos, mesh, rel_l2, rel_linf
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from lpm_amd import _abi  # noqa: E402
+
+OMEGA = 2 * np.pi
+TOL64 = 1e-12
+TOL_MIXED = 1e-6
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from lpm_amd.device import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def dev(a, dtype=torch.float64):
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def rh(x):
+    from lpm_amd import galleries as G
+    return G.rh54_vorticity(x, 0.1, -0.1)
+
+
+def _inputs(seed, depth):
+    m = mesh(seed, depth)
+    z = rh(m.face_x)
+    return m, z
+
+
+@pytest.mark.parametrize("seed,depth", [(0, 3), (1, 3), (0, 4)])
+def test_velocity_fp64_vertices_match_oracle(ctx, ora, seed, depth):
+    m, z = _inputs(seed, depth)
+    u = ctx.bve_velocity(dev(m.vert_x), dev(m.face_x), dev(z), dev(m.face_area),
+                         dev(m.face_mask, torch.uint8))
+    ref = ora.bve_velocity(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                           m.face_mask)
+    got = u.cpu().numpy().T
+    assert rel_l2(got, ref) < TOL64
+    assert rel_linf(got, ref) < TOL64
+
+
+@pytest.mark.parametrize("seed,depth", [(0, 3), (1, 3), (1, 4)])
+def test_velocity_fp64_faces_match_oracle(ctx, ora, seed, depth):
+    m, z = _inputs(seed, depth)
+    fx = dev(m.face_x)
+    u = ctx.bve_velocity(fx, fx, dev(z), dev(m.face_area),
+                         dev(m.face_mask, torch.uint8), collocated=True)
+    ref = ora.bve_velocity(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                           m.face_mask, collocated=True)
+    got = u.cpu().numpy().T
+    leaf = m.face_mask == 0
+    assert rel_l2(got[leaf], ref[leaf]) < TOL64
+    assert rel_linf(got[leaf], ref[leaf]) < TOL64
+    if seed == 1:
+        # cubed sphere: no divided face coincides with a source, all finite
+        assert rel_l2(got, ref) < TOL64
+    else:
+        # icosahedral seed faces share their central child's centre: the
+        # reference's parent-face velocity is singular, so is ours
+        assert not np.all(np.isfinite(got[:20]))
+
+
+def test_strided_layouts_and_unmasked_sources(ctx):
+    m, z = _inputs(1, 3)
+    vx = dev(m.vert_x)
+    fz, fa = dev(z), dev(m.face_area)
+    fm = dev(m.face_mask, torch.uint8)
+    soa = ctx.bve_velocity(vx, dev(m.face_x), fz, fa, fm)
+    # AoS (n,3) buffers passed as transposed views: element stride 3
+    vx_aos = dev(aos(m.vert_x)).T
+    fx_aos = dev(aos(m.face_x)).T
+    out_aos = torch.zeros((m.n_verts, 3), dtype=torch.float64, device="cuda").T
+    ctx.bve_velocity(vx_aos, fx_aos, fz, fa, fm, out=out_aos)
+    assert torch.equal(soa, out_aos)
+    # leaf faces only, no mask: same sources, same order -> identical
+    ls = m.leaf_slice
+    leaf_only = ctx.bve_velocity(vx, dev(m.face_x[:, ls]), dev(z[ls]),
+                                 dev(m.face_area[ls]), None)
+    assert torch.equal(soa, leaf_only)
+
+
+def test_velocity_mixed_matches_fp64(ctx):
+    for seed, depth in [(0, 5), (1, 5)]:
+        m, z = _inputs(seed, depth)
+        vx, fx = dev(m.vert_x), dev(m.face_x)
+        args = (dev(z), dev(m.face_area), dev(m.face_mask, torch.uint8))
+        u64 = ctx.bve_velocity(vx, fx, *args)
+        u32 = ctx.bve_velocity(vx, fx, *args, precision="mixed")
+        e = rel_l2(u32.cpu().numpy(), u64.cpu().numpy())
+        assert e < TOL_MIXED, e
+        f64 = ctx.bve_velocity(fx, fx, *args, collocated=True)
+        f32 = ctx.bve_velocity(fx, fx, *args, collocated=True, precision="mixed")
+        leaf = torch.from_numpy(m.face_mask == 0)
+        e = rel_l2(f32[:, leaf].cpu().numpy(), f64[:, leaf].cpu().numpy())
+        assert e < TOL_MIXED, e
+
+
+def test_stream_function_matches_oracle(ctx, ora):
+    m, z = _inputs(0, 3)
+    psi = ctx.bve_stream_fn(dev(m.vert_x), dev(m.face_x), dev(z),
+                            dev(m.face_area), dev(m.face_mask, torch.uint8))
+    ref = ora.bve_stream_fn(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                            m.face_mask)
+    assert rel_l2(psi.cpu().numpy(), ref) < TOL64
+    fx = dev(m.face_x)
+    psi = ctx.bve_stream_fn(fx, fx, dev(z), dev(m.face_area),
+                            dev(m.face_mask, torch.uint8), collocated=True)
+    ref = ora.bve_stream_fn(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                            m.face_mask, collocated=True)
+    leaf = m.face_mask == 0
+    assert rel_l2(psi.cpu().numpy()[leaf], ref[leaf]) < TOL64
+
+
+def test_skip_masked_targets_leaves_them_untouched(ctx):
+    m, z = _inputs(0, 3)
+    fx = dev(m.face_x)
+    out = torch.full((3, m.n_faces), 7.0, dtype=torch.float64, device="cuda")
+    ctx.bve_velocity(fx, fx, dev(z), dev(m.face_area),
+                     dev(m.face_mask, torch.uint8), collocated=True,
+                     flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS, out=out)
+    full = ctx.bve_velocity(fx, fx, dev(z), dev(m.face_area),
+                            dev(m.face_mask, torch.uint8), collocated=True)
+    div = torch.from_numpy(m.face_mask == 1)
+    assert torch.all(out[:, div] == 7.0)
+    assert torch.equal(out[:, ~div], full[:, ~div])
+
+
+def test_edge_cases(ctx, ora):
+    rng = np.random.default_rng(11)
+    for ns in (1, 3, 255, 256, 257, 1000):
+        y = rng.normal(size=(3, ns))
+        y /= np.linalg.norm(y, axis=0)
+        x = rng.normal(size=(3, 37))
+        x /= np.linalg.norm(x, axis=0)
+        zz = rng.normal(size=ns)
+        aa = rng.uniform(0.1, 1.0, size=ns)
+        for prec in ("fp64", "mixed"):
+            u = ctx.bve_velocity(dev(x), dev(y), dev(zz), dev(aa), None,
+                                 precision=prec)
+            ref = ora.bve_velocity(aos(x), aos(y), zz, aa)
+            tol = TOL64 if prec == "fp64" else 1e-5
+            assert rel_l2(u.cpu().numpy().T, ref) < tol, (ns, prec)
+    # no targets
+    y = dev(np.eye(3))
+    out = ctx.bve_velocity(torch.zeros((3, 0), dtype=torch.float64, device="cuda"),
+                           y, dev(np.ones(3)), dev(np.ones(3)), None)
+    assert out.shape == (3, 0)
+    # every source divided: zero velocity
+    x = dev(np.array([[1.0], [0.0], [0.0]]))
+    u = ctx.bve_velocity(x, y, dev(np.ones(3)), dev(np.ones(3)),
+                         dev(np.ones(3), torch.uint8))
+    assert torch.all(u == 0)
+
+
+def test_argument_errors(ctx):
+    L = _abi.lib()
+    x = dev(np.eye(3))
+    p = x.data_ptr()
+    one = dev(np.ones(3))
+    # negative size
+    rc = L.lpm_bve_velocity(ctx.h, p, 1, 3, -1, p, 1, 3, one.data_ptr(),
+                            one.data_ptr(), None, 3, 0, 0.0, 0, 0, p, 1, 3, None)
+    assert rc == _abi.LPM_ERR_ARG
+    # collocated with n_tgt != n_src
+    rc = L.lpm_bve_velocity(ctx.h, p, 1, 3, 2, p, 1, 3, one.data_ptr(),
+                            one.data_ptr(), None, 3, 1, 0.0, 0, 0, p, 1, 3, None)
+    assert rc == _abi.LPM_ERR_ARG
+    assert b"collocated" in L.lpm_last_error(ctx.h)
+    # unknown precision
+    rc = L.lpm_bve_velocity(ctx.h, p, 1, 3, 3, p, 1, 3, one.data_ptr(),
+                            one.data_ptr(), None, 3, 0, 0.0, 9, 0, p, 1, 3, None)
+    assert rc == _abi.LPM_ERR_ARG
+
+
+def test_nonfinite_is_reported(ctx):
+    m, z = _inputs(1, 2)
+    z = z.copy()
+    z[m.n_faces - 1] = np.nan
+    with pytest.raises(_abi.LpmError) as ei:
+        ctx.bve_velocity(dev(m.vert_x), dev(m.face_x), dev(z), dev(m.face_area),
+                         dev(m.face_mask, torch.uint8),
+                         flags=_abi.LPM_FLAG_CHECK_FINITE)
+    assert ei.value.code == _abi.LPM_ERR_NONFINITE
+
+
+# ------------------------------------------------------------------ RK4
+def _sphere(ctx, seed, depth, zeta_fn=rh):
+    from lpm_amd.bve import BVESphere
+    s = BVESphere(seed, depth, ctx=ctx)
+    s.init_vorticity(zeta_fn)
+    return s
+
+
+def _oracle_state(s):
+    m = s.mesh
+    return {"vx": aos(s.vert_x.cpu().numpy()), "vzeta": s.rel_vort_verts.cpu().numpy().copy(),
+            "vu": aos(s.velocity_verts.cpu().numpy()),
+            "fx": aos(s.face_x.cpu().numpy()), "fzeta": s.rel_vort_faces.cpu().numpy().copy(),
+            "fu": aos(s.velocity_faces.cpu().numpy()),
+            "farea": m.face_area.copy(), "fmask": m.face_mask.copy()}
+
+
+@pytest.mark.parametrize("compat,renorm", [(True, False), (False, False), (False, True)])
+def test_rk4_steps_match_oracle(ctx, ora, compat, renorm):
+    from lpm_amd.bve import BVERK4
+    s = _sphere(ctx, 1, 3)
+    s.init_velocity()
+    st = _oracle_state(s)
+    solver = BVERK4(0.01, s, reference_compat=compat, renormalize=renorm)
+    for _ in range(3):
+        solver.advance_timestep(s)
+        ora.bve_rk4_step(st, 0.01, OMEGA, compat=compat, renorm=renorm)
+    torch.cuda.synchronize()
+    assert rel_l2(aos(s.vert_x.cpu().numpy()), st["vx"]) < TOL64
+    assert rel_l2(aos(s.face_x.cpu().numpy()), st["fx"]) < TOL64
+    assert rel_l2(s.rel_vort_verts.cpu().numpy(), st["vzeta"]) < TOL64
+    assert rel_l2(s.rel_vort_faces.cpu().numpy(), st["fzeta"]) < TOL64
+    assert rel_l2(aos(s.velocity_verts.cpu().numpy()), st["vu"]) < 1e-11
+    assert rel_l2(aos(s.velocity_faces.cpu().numpy()), st["fu"]) < 1e-11
+
+
+def test_rk4_graph_replays_match_eager(ctx):
+    from lpm_amd.bve import BVERK4
+    a = _sphere(ctx, 0, 3)
+    b = _sphere(ctx, 0, 3)
+    flags = dict(reference_compat=True, leaf_only=True)
+    a.init_velocity(flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    b.init_velocity(flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    ra = BVERK4(0.01, a, **flags)
+    rb = BVERK4(0.01, b, use_graph=True, **flags)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for k in range(4):
+            ra.advance_timestep(a)
+            rb.advance_timestep(b)
+            # eager work of the program between replays
+            torch.cuda.synchronize()
+            _ = (a.vert_x * 2).sum().item()
+            assert torch.equal(a.vert_x, b.vert_x), k
+            assert torch.equal(a.face_x, b.face_x), k
+            assert torch.equal(a.rel_vort_faces, b.rel_vort_faces), k
+            assert torch.equal(a.velocity_verts, b.velocity_verts), k
+
+
+def test_rk4_solid_body_rotation(ctx):
+    from lpm_amd.bve import BVERK4
+    Om_r = 0.5
+    s = _sphere(ctx, 0, 4, lambda x: 2 * Om_r * x[2])
+    s.init_velocity(flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    x0 = s.vert_x.cpu().numpy().copy()
+    solver = BVERK4(0.05, s, leaf_only=True, precision="mixed", renormalize=True)
+    for _ in range(8):
+        solver.advance_timestep(s)
+    ang = Om_r * 0.4
+    c, sn = np.cos(ang), np.sin(ang)
+    ex = np.stack([c * x0[0] - sn * x0[1], sn * x0[0] + c * x0[1], x0[2]])
+    err = np.max(np.linalg.norm(s.vert_x.cpu().numpy() - ex, axis=0))
+    assert err < 1e-2 * ang
+
+
+def test_timing_records_launches(ctx):
+    m, z = _inputs(0, 3)
+    ctx.timing_enable(True)
+    for _ in range(3):
+        ctx.bve_velocity(dev(m.vert_x), dev(m.face_x), dev(z), dev(m.face_area),
+                         dev(m.face_mask, torch.uint8))
+    ms, n, pairs = ctx.timing_query(_abi.LPM_KERNEL_BVE_VEL_F64)
+    ctx.timing_enable(False)
+    assert n == 3 and ms > 0
+    assert pairs == 3 * m.n_verts * m.n_faces
