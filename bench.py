@@ -48,7 +48,8 @@ def errors(a, b):
     return float(np.sqrt((e ** 2).sum() / (r ** 2).sum())), float(e.max() / r.max())
 
 
-def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra):
+def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra,
+                  autotune=True):
     import numpy as np
     import torch
     from lpm_amd import _abi
@@ -58,6 +59,20 @@ def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra):
     s = BVESphere("icos", mesh.depth, ctx=ctx, mesh=mesh)
     s.init_vorticity(lambda x: G.rh54_vorticity(x, 0.1, -0.1))
     skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    kid = (_abi.LPM_KERNEL_BVE_VEL_F64 if precision == "fp64"
+           else _abi.LPM_KERNEL_BVE_VEL_MIXED)
+    tune = None
+    if autotune:
+        names = ctx.variants(kid)
+        best, best_ms, all_ms = ctx.bve_autotune(s.fields(), precision,
+                                                 flags=skip, repeats=1)
+        d = ctx.L.lpm_variant_default(kid)
+        tune = {"variant": names[best], "ms": round(best_ms, 3),
+                "fixed_tile_baseline": {"variant": names[d],
+                                        "ms": round(all_ms[d], 3)},
+                "all_ms": {n: round(t, 3) for n, t in zip(names, all_ms)}}
+        log(f"autotune {precision}: {names[best]} {best_ms:.2f} ms "
+            f"(fixed {names[d]} {all_ms[d]:.2f} ms)")
     s.init_velocity(precision, flags=skip)
     torch.cuda.synchronize()
     u0_v = s.velocity_verts.cpu().numpy()
@@ -66,8 +81,6 @@ def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra):
     for _ in range(warmup):
         solver.advance_timestep(s)
     torch.cuda.synchronize()
-    kid = (_abi.LPM_KERNEL_BVE_VEL_F64 if precision == "fp64"
-           else _abi.LPM_KERNEL_BVE_VEL_MIXED)
     ctx.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -79,6 +92,7 @@ def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra):
     ctx.timing_enable(False)
     elapsed = t1 - t0
     out = {"elapsed_s": elapsed, "kernel_ms_total": kms, "launches": launches,
+           "autotune": tune,
            "u0_v": u0_v, "u0_f": u0_f,
            "uT_v": s.velocity_verts.cpu().numpy(),
            "xT_v": s.vert_x.cpu().numpy()}
@@ -140,6 +154,8 @@ def main(argv=None):
                     help="seconds of CPU oracle work for the baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the fixed default kernel variants")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -165,7 +181,8 @@ def main(argv=None):
 
     log("FP64 run")
     r64 = run_precision(ctx, mesh, "fp64", a.steps, a.warmup, a.dt,
-                        dict(reference_compat=True))
+                        dict(reference_compat=True),
+                        autotune=not a.no_autotune)
     step_s = r64["elapsed_s"] / a.steps
     value = 4 * pairs_eval * a.steps / r64["elapsed_s"] / 1e9
     avg_ms = r64["kernel_ms_total"] / max(r64["launches"], 1)
@@ -214,17 +231,20 @@ def main(argv=None):
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
             "traffic": None,
             "kernel": "bve_vel_f64_kernel",
+            "variant": (r64["autotune"] or {}).get("variant"),
             "avg_launch_ms": round(avg_ms, 4),
             "flops_per_pair": F_BVE,
             "pairs_per_launch": pairs_eval,
         },
         "device": {"name": props.name, "cus": props.multi_processor_count},
+        "autotune_fp64": r64["autotune"],
     }
 
     if not a.no_mixed:
         log("mixed run")
         rmx = run_precision(ctx, mesh, "mixed", a.steps, a.warmup, a.dt,
-                            dict(reference_compat=True, renormalize=True))
+                            dict(reference_compat=True, renormalize=True),
+                            autotune=not a.no_autotune)
         mstep = rmx["elapsed_s"] / a.steps
         mval = 4 * pairs_eval * a.steps / rmx["elapsed_s"] / 1e9
         mavg = rmx["kernel_ms_total"] / max(rmx["launches"], 1)
@@ -243,6 +263,7 @@ def main(argv=None):
             "velocity_error_vs_fp64_after_steps_vertices_rel_l2_linf":
                 errors(rmx["uT_v"], r64["uT_v"]),
             "vs_analytic_rh54_vertices_rel_l2_linf": errors(rmx["u0_v"], ue_v),
+            "autotune": rmx["autotune"],
         }
         log(f"mixed: {mstep*1e3:.1f} ms/step, {mval:.1f} Gpairs/s "
             f"({mval/value:.2f}x FP64)")

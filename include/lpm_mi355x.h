@@ -184,6 +184,106 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                      double Omega, int precision, unsigned flags,
                      void* stream);
 
+/* --------------------------------------------------------- SWE sums ---- */
+/* Sphere shallow-water direct sums at the targets: velocity from relative
+ * vorticity zeta and divergence sigma,
+ *   u(x) = sum_j kzeta(x, y_j) zeta_j A_j + ksigma(x, y_j) sigma_j A_j,
+ * the double dot ddot = grad(u):grad(u)^T of the packed velocity gradient the
+ * reference forms, and the Gaussian PSE surface Laplacian
+ *   laps(x_i) = eps_p^-4 sum_j [x_i.y_j > 0] (s_j - s_i) A_j (4/pi)
+ *               exp(-|x_i - y_j|^2 / eps_p^2).
+ * Replaces SphereVertexSums (collocated = 0) / SphereFaceSums (collocated = 1)
+ * with SphereSweDirectSumReducer, sphere_swe_velocity_sums, kzeta_sphere,
+ * ksigma_sphere, grad_kzeta, grad_ksigma (src/lpm_swe_kernels.hpp:62-362,
+ * 578-619, 723-780, 877-930) plus LaplacianReducer / BivariateOrder2
+ * (src/lpm_pse.hpp:75-111, 157-203).  The reference's sphere PSE branch is
+ * unreachable (its kernels use planar distances); chord distances are used.
+ * Collocated sums skip j == i only when eps == 0, as SphereFaceSums does.
+ * pse_eps == 0 disables the Laplacian (out_laps untouched).  Any output may
+ * be NULL. */
+int lpm_swe_sphere_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                        int64_t tgt_cs, int64_t n_tgt, const double* tgt_surf,
+                        const double* src_x, int64_t src_es, int64_t src_cs,
+                        const double* src_zeta, const double* src_sigma,
+                        const double* src_area, const uint8_t* src_mask,
+                        const double* src_surf, int64_t n_src, int collocated,
+                        double eps, double pse_eps, int precision,
+                        unsigned flags, double* out_u, int64_t out_es,
+                        int64_t out_cs, double* out_ddot, double* out_laps,
+                        void* stream);
+
+/* Fields of SWE<Seed> (src/lpm_swe.hpp:18-177) the sphere stepper reads and
+ * writes.  Bottom topography is zero, so surface = depth. */
+typedef struct lpm_swe_fields {
+  int64_t n_verts;
+  double* vert_x;
+  int64_t vert_x_es, vert_x_cs;
+  double* vert_zeta;   /* state: relative vorticity */
+  double* vert_sigma;  /* state: divergence */
+  double* vert_h;      /* state: depth */
+  double* vert_u;      /* velocity */
+  int64_t vert_u_es, vert_u_cs;
+  double* vert_ddot;   /* grad(u):grad(u)^T */
+  double* vert_laps;   /* surface Laplacian */
+  double* vert_surf;   /* surface height */
+  int64_t n_faces;
+  double* face_x;
+  int64_t face_x_es, face_x_cs;
+  double* face_zeta;
+  double* face_sigma;
+  double* face_area;   /* state: area (evolves, dA/dt = sigma A) */
+  const double* face_mass; /* fixed mass = depth * area */
+  double* face_h;      /* derived depth = mass / area */
+  double* face_u;
+  int64_t face_u_es, face_u_cs;
+  double* face_ddot;
+  double* face_laps;
+  double* face_surf;
+  const uint8_t* face_mask;
+} lpm_swe_fields;
+
+/* u, ddot and laps at the current state (SWE::init_direct_sums,
+ * src/lpm_swe_impl.hpp:401-445, plus the PSE Laplacian). */
+int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
+                      double pse_eps, int precision, unsigned flags,
+                      void* stream);
+
+/* One RK4 step of (x, zeta, sigma, h | A) in the stage layout of
+ * SWERK4::advance_timestep (src/lpm_swe_rk4_impl.hpp:204-445; update
+ * SWERK4Update :12-87; tendencies SWEVorticityDivergenceHeightTendencies /
+ * ...AreaTendencies src/lpm_swe_kernels.hpp:941-1069 with CoriolisSphere
+ * src/lpm_coriolis.hpp:154-195), with the sphere sums.  On entry u, ddot,
+ * laps and surf must be current (lpm_swe_init_sums or the previous step).
+ * LPM_RK4_REFERENCE_COMPAT: the stage-4 position increment stays zero, as
+ * in the reference, which never assigns it.  LPM_FLAG_SKIP_MASKED_TARGETS:
+ * divided faces are neither evaluated nor advanced. */
+int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
+                     double Omega, double g, double eps, double pse_eps,
+                     int precision, unsigned flags, void* stream);
+
+/* ---------------------------------------------------- kernel variants ---- */
+/* The direct-sum kernels are compiled in several tile/block/unroll shapes
+ * ("variants").  No reference counterpart (Kokkos picks TeamPolicy(n, AUTO),
+ * src/lpm_bve_rk4_impl.hpp:70-71).  kernel_id is LPM_KERNEL_*. */
+int lpm_variant_count(int kernel_id);
+const char* lpm_variant_name(int kernel_id, int variant);
+/* The fixed variant used when nothing was tuned or forced. */
+int lpm_variant_default(int kernel_id);
+/* Forces `variant` for every problem size; -1 returns to the tuned/default
+ * choice. */
+int lpm_set_variant(lpm_ctx* ctx, int kernel_id, int variant);
+/* Autotuner: times every variant on the velocity evaluation of `f` (it
+ * overwrites vert_u/face_u with the velocity at the current state, as
+ * lpm_bve_init_velocity does), `repeats` launches each, and records the
+ * fastest for this device and problem-size bucket (floor(log2(targets))).
+ * all_ms_host (nullable) receives lpm_variant_count() per-variant times. */
+int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
+                     unsigned flags, int repeats, int* best_variant,
+                     double* best_ms, double* all_ms_host, void* stream);
+/* Tuning table persistence (text: "arch/kernel/bucket variant" lines). */
+int lpm_tune_save(lpm_ctx* ctx, const char* path);
+int lpm_tune_load(lpm_ctx* ctx, const char* path);
+
 /* ------------------------------------------------------------ timing ---- */
 /* HIP events recorded around every direct-sum launch, on the stream the
  * kernel is launched on (disabled inside graph capture).  Counterpart of the

@@ -69,6 +69,23 @@ class BveFields(ctypes.Structure):
     ]
 
 
+class SweFields(ctypes.Structure):
+    """struct lpm_swe_fields."""
+    _fields_ = [
+        ("n_verts", c_i64), ("vert_x", c_vp), ("vert_x_es", c_i64),
+        ("vert_x_cs", c_i64), ("vert_zeta", c_vp), ("vert_sigma", c_vp),
+        ("vert_h", c_vp), ("vert_u", c_vp), ("vert_u_es", c_i64),
+        ("vert_u_cs", c_i64), ("vert_ddot", c_vp), ("vert_laps", c_vp),
+        ("vert_surf", c_vp),
+        ("n_faces", c_i64), ("face_x", c_vp), ("face_x_es", c_i64),
+        ("face_x_cs", c_i64), ("face_zeta", c_vp), ("face_sigma", c_vp),
+        ("face_area", c_vp), ("face_mass", c_vp), ("face_h", c_vp),
+        ("face_u", c_vp), ("face_u_es", c_i64), ("face_u_cs", c_i64),
+        ("face_ddot", c_vp), ("face_laps", c_vp), ("face_surf", c_vp),
+        ("face_mask", c_vp),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "lpm_abi_version": (c_int, []),
@@ -93,6 +110,22 @@ _SIGS = {
                                       c_uint, c_vp]),
     "lpm_bve_rk4_step": (c_int, [c_vp, ctypes.POINTER(BveFields), c_dbl, c_dbl,
                                  c_int, c_uint, c_vp]),
+    "lpm_swe_sphere_sums": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp,
+                                    c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                    c_i64, c_int, c_dbl, c_dbl, c_int, c_uint,
+                                    c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "lpm_swe_init_sums": (c_int, [c_vp, ctypes.POINTER(SweFields), c_dbl, c_dbl,
+                                  c_int, c_uint, c_vp]),
+    "lpm_swe_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweFields), c_dbl, c_dbl,
+                                 c_dbl, c_dbl, c_dbl, c_int, c_uint, c_vp]),
+    "lpm_variant_count": (c_int, [c_int]),
+    "lpm_variant_default": (c_int, [c_int]),
+    "lpm_variant_name": (ctypes.c_char_p, [c_int, c_int]),
+    "lpm_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "lpm_bve_autotune": (c_int, [c_vp, ctypes.POINTER(BveFields), c_int, c_uint,
+                                 c_int, c_intp, c_dblp, c_dblp, c_vp]),
+    "lpm_tune_save": (c_int, [c_vp, ctypes.c_char_p]),
+    "lpm_tune_load": (c_int, [c_vp, ctypes.c_char_p]),
     "lpm_timing_enable": (c_int, [c_vp, c_int]),
     "lpm_timing_query": (c_int, [c_vp, c_int, c_dblp, c_i64p, c_dblp]),
 }

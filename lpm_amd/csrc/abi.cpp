@@ -5,6 +5,7 @@
 // rethrow).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstring>
 #include <new>
 
@@ -76,6 +77,21 @@ void* DevBuf::ensure(size_t nbytes) {
   bytes = want;
   g_workspace_epoch.fetch_add(1);
   return ptr;
+}
+
+// Size bucket: floor(log2(total targets)); tables are per device arch.
+std::string tune_key(const lpm_ctx* ctx, int family, int64_t n_targets) {
+  int b = 0;
+  while ((int64_t(1) << (b + 1)) <= n_targets && b < 62) ++b;
+  return ctx->arch + "/" + std::to_string(family) + "/" + std::to_string(b);
+}
+
+int pick_variant(lpm_ctx* ctx, int family, int64_t n_targets) {
+  if (ctx->forced[family] >= 0) return ctx->forced[family];
+  auto it = ctx->tuned.find(tune_key(ctx, family, n_targets));
+  if (it != ctx->tuned.end() && it->second < variant_count(family))
+    return it->second;
+  return default_variant(family);
 }
 
 TimedLaunch::TimedLaunch(lpm_ctx* c, int k, hipStream_t s, double p)
@@ -322,6 +338,51 @@ int lpm_bve_stream_fn(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
     launch_bve_stream_fn(ctx, ps, &t, 1, s);
     if (flags & LPM_FLAG_CHECK_FINITE)
       check_finite(ctx, out_psi, n_tgt, 1, 1, 1, s);
+  });
+}
+
+// ----------------------------------------------------------- variants
+int lpm_variant_count(int kernel_id) { return variant_count(kernel_id); }
+
+int lpm_variant_default(int kernel_id) {
+  return variant_count(kernel_id) > 0 ? default_variant(kernel_id) : -1;
+}
+
+const char* lpm_variant_name(int kernel_id, int variant) {
+  const char* n = variant_name(kernel_id, variant);
+  return n ? n : "";
+}
+
+int lpm_set_variant(lpm_ctx* ctx, int kernel_id, int variant) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(kernel_id >= 0 && kernel_id < KID_COUNT, "bad kernel id");
+    LPM_ARG_CHECK(variant >= -1 && variant < variant_count(kernel_id),
+                  "variant out of range");
+    ctx->forced[kernel_id] = variant;
+  });
+}
+
+int lpm_tune_save(lpm_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx && path, "null argument");
+    FILE* f = std::fopen(path, "w");
+    LPM_ARG_CHECK(f != nullptr, std::string("cannot write ") + path);
+    for (const auto& kv : ctx->tuned)
+      std::fprintf(f, "%s %d\n", kv.first.c_str(), kv.second);
+    std::fclose(f);
+  });
+}
+
+int lpm_tune_load(lpm_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx && path, "null argument");
+    FILE* f = std::fopen(path, "r");
+    LPM_ARG_CHECK(f != nullptr, std::string("cannot read ") + path);
+    char key[256];
+    int v = 0;
+    while (std::fscanf(f, "%255s %d", key, &v) == 2) ctx->tuned[key] = v;
+    std::fclose(f);
   });
 }
 

@@ -310,6 +310,71 @@ int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
   }
 }
 
+int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
+                     unsigned flags, int repeats, int* best_variant,
+                     double* best_ms, double* all_ms, void* stream) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int family = precision == LPM_MIXED ? KID_BVE_VEL_MIXED : KID_BVE_VEL_F64;
+  int saved = -1;
+  bool timing = false;
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(repeats >= 1 && repeats <= 100, "repeats must be in [1, 100]");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    saved = ctx->forced[family];
+    timing = ctx->timing;
+    ctx->timing = false;
+    LPM_HIP_CHECK(hipEventCreate(&e0));
+    LPM_HIP_CHECK(hipEventCreate(&e1));
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    int best = -1;
+    float best_t = 0;
+    for (int v = 0; v < variant_count(family); ++v) {
+      ctx->forced[family] = v;
+      velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
+                    f->face_x_es, f->face_x_cs, f->face_zeta, map, precision,
+                    flags, s);
+      LPM_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < repeats; ++r)
+        velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs,
+                      f->face_x, f->face_x_es, f->face_x_cs, f->face_zeta, map,
+                      precision, flags, s);
+      LPM_HIP_CHECK(hipEventRecord(e1, s));
+      LPM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      LPM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      ms /= repeats;
+      if (all_ms) all_ms[v] = ms;
+      if (best < 0 || ms < best_t) {
+        best = v;
+        best_t = ms;
+      }
+    }
+    ctx->forced[family] = saved;
+    ctx->timing = timing;
+    ctx->tuned[tune_key(ctx, family, f->n_verts + f->n_faces)] = best;
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return LPM_OK;
+  } catch (const std::exception& e) {
+    if (ctx) {
+      ctx->forced[family] = saved;
+      ctx->timing = timing;
+      set_error(ctx, e.what());
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const Error* le = dynamic_cast<const Error*>(&e);
+    return le ? le->code : LPM_ERR_INTERNAL;
+  }
+}
+
 int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                      double Omega, int precision, unsigned flags,
                      void* stream) {

@@ -139,8 +139,12 @@ struct lpm_ctx {
   lpm_amd::DevBuf flag;         // int32 non-finite flag
   lpm_amd::DevBuf count_ident;  // int32 count when no mask
 
+  lpm_amd::DevBuf swe_packed;   // SWE sources (64 B each)
+  lpm_amd::DevBuf swe_bounds;   // SWE tile bounding spheres
+
   // RK4 stage buffers
-  lpm_amd::DevBuf rk_v, rk_f;   // vertex / face stage storage
+  lpm_amd::DevBuf rk_v, rk_f;   // BVE vertex / face stage storage
+  lpm_amd::DevBuf swe_v, swe_f; // SWE vertex / face stage storage
 
   // timing
   bool timing = false;
@@ -151,13 +155,21 @@ struct lpm_ctx {
   struct GraphEntry;
   std::map<std::string, std::shared_ptr<GraphEntry>> graphs;
 
-  // autotuned kernel variants, keyed by (kernel, size bucket)
+  // autotuned kernel variants, keyed by (arch, kernel, size bucket)
   std::map<std::string, int> tuned;
+  int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1};
 };
 
 namespace lpm_amd {
 
 void set_error(lpm_ctx* ctx, const std::string& msg);
+
+// Kernel variants of a family (KID_*) and the autotuning table.
+int variant_count(int family);
+const char* variant_name(int family, int v);
+int default_variant(int family);
+std::string tune_key(const lpm_ctx* ctx, int family, int64_t n_targets);
+int pick_variant(lpm_ctx* ctx, int family, int64_t n_targets);
 hipStream_t as_stream(void* s);
 
 // Event bracket for a timed launch (no-op when timing is off).
@@ -188,5 +200,26 @@ void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s);
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
                   int64_t cs, int ncomp, hipStream_t s);
+
+// ---- sphere SWE sums (swe_sums.hip) ----
+struct SwePacked {
+  const void* data = nullptr;       // 64 B per source
+  const double4* bounds = nullptr;  // per 256-source tile: centre, radius^2
+  const int32_t* count = nullptr;
+  int64_t capacity = 0;
+};
+struct SweTargets {
+  TargetSeg base;                   // positions, slots, self, velocity out
+  const double* surf = nullptr;     // target surface height (PSE)
+  double* ddot = nullptr;
+  double* laps = nullptr;
+};
+SwePacked swe_pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
+                           const double* sigma, const double* area,
+                           const double* surf, const RankMap& map,
+                           hipStream_t s);
+void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
+                     int nseg, double eps, double pse_eps, int precision,
+                     hipStream_t s);
 
 }  // namespace lpm_amd

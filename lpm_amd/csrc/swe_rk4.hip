@@ -1,0 +1,453 @@
+// Sphere SWE entry points: direct sums and the RK4 stepper.  The stepper
+// follows the stage layout of SWERK4::advance_timestep
+// (src/lpm_swe_rk4_impl.hpp:204-445) with the sphere sums and the sphere
+// branch of the tendency functors (src/lpm_swe_kernels.hpp:941-1069), zero
+// bottom topography (SetSurfaceFromDepth / SetDepthAndSurfaceFromMassAndArea,
+// :1079-1140).  Per stage, one fused kernel per particle kind evaluates the
+// tendencies, stores the stage increments and forms the next stage's work
+// state with its surface.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace lpm_amd {
+
+namespace {
+
+// Stage storage of one particle kind (SoA, leading dimension n):
+//   k[s] (3n), zk[s], sk[s], hk[s] (n each) for s = 0..3;
+//   xw (3n), zw, sw, hw (n each).
+struct SweStage {
+  double* k[4];
+  double* zk[4];
+  double* sk[4];
+  double* hk[4];
+  double* xw;
+  double* zw;
+  double* sw;
+  double* hw;
+  int64_t n;
+};
+
+SweStage carve(DevBuf& buf, int64_t n) {
+  const int64_t nn = std::max<int64_t>(n, 1);
+  double* p = static_cast<double*>(buf.ensure(sizeof(double) * 30 * nn));
+  SweStage b;
+  for (int s = 0; s < 4; ++s) {
+    b.k[s] = p + (6 * s) * nn;
+    b.zk[s] = p + (6 * s + 3) * nn;
+    b.sk[s] = p + (6 * s + 4) * nn;
+    b.hk[s] = p + (6 * s + 5) * nn;
+  }
+  b.xw = p + 24 * nn;
+  b.zw = p + 27 * nn;
+  b.sw = p + 28 * nn;
+  b.hw = p + 29 * nn;
+  b.n = n;
+  return b;
+}
+
+struct SweStageArgs {
+  // stage input (state for stage 0, work arrays afterwards)
+  const double* x;
+  int64_t xes, xcs;
+  const double* zeta;
+  const double* sigma;
+  const double* ha;  // depth (vertices) or area (faces)
+  // sums at the stage input
+  const double* u;
+  int64_t ues, ucs;
+  const double* ddot;
+  const double* laps;
+  // state (for the next work state)
+  const double* x0;
+  int64_t x0es, x0cs;
+  const double* zeta0;
+  const double* sigma0;
+  const double* ha0;
+  // outputs
+  double* k;
+  double* zk;
+  double* sk;
+  double* hk;
+  double* xw;  // nullptr after the last stage
+  double* zw;
+  double* sw;
+  double* hw;
+  double* surf;       // surface of the next work state
+  double* face_h;     // faces: depth = mass / area of the next work state
+  const double* mass;
+  const uint8_t* skip;
+  const uint8_t* mask;  // faces: divided faces keep h, s
+  int64_t n;
+  double dt, Omega, g, a;
+  int is_face;
+  int store_k;  // 0: leave k[s] (position increment) at zero
+};
+
+// Tendencies (sphere branch), stage increments and the next work state.
+__global__ __launch_bounds__(256) void swe_stage_kernel(SweStageArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.skip && A.skip[i]) return;
+  const int64_t n = A.n;
+  const double* xi = A.x + i * A.xes;
+  const double x0 = xi[0], x1 = xi[A.xcs], x2 = xi[2 * A.xcs];
+  const double* ui = A.u + i * A.ues;
+  const double u0 = ui[0], u1 = ui[A.ucs], u2 = ui[2 * A.ucs];
+  const double zeta = A.zeta[i], sigma = A.sigma[i], ha = A.ha[i];
+  const double f = 2.0 * A.Omega * x2;
+  const double dfdt = 2.0 * A.Omega * u2;
+  const double gfxu = -2.0 * A.Omega * (-u0 * x1 + u1 * x0);
+  const double dz = (-dfdt - (zeta + f) * sigma) * A.dt;
+  const double ds = (f * zeta + gfxu - A.ddot[i] - A.g * A.laps[i] -
+                     (u0 * u0 + u1 * u1 + u2 * u2)) * A.dt;
+  const double dh = A.is_face ? (sigma * ha) * A.dt : (-sigma * ha) * A.dt;
+  const double k0 = A.store_k ? A.dt * u0 : 0.0;
+  const double k1 = A.store_k ? A.dt * u1 : 0.0;
+  const double k2 = A.store_k ? A.dt * u2 : 0.0;
+  A.k[i] = k0;
+  A.k[i + n] = k1;
+  A.k[i + 2 * n] = k2;
+  A.zk[i] = dz;
+  A.sk[i] = ds;
+  A.hk[i] = dh;
+  if (A.xw) {
+    const double* y = A.x0 + i * A.x0es;
+    A.xw[i] = y[0] + A.a * k0;
+    A.xw[i + n] = y[A.x0cs] + A.a * k1;
+    A.xw[i + 2 * n] = y[2 * A.x0cs] + A.a * k2;
+    A.zw[i] = A.zeta0[i] + A.a * dz;
+    A.sw[i] = A.sigma0[i] + A.a * ds;
+    const double hw = A.ha0[i] + A.a * dh;
+    A.hw[i] = hw;
+    if (!A.is_face) {
+      A.surf[i] = hw;
+    } else if (!(A.mask && A.mask[i])) {
+      const double h = A.mass[i] / hw;
+      A.face_h[i] = h;
+      A.surf[i] = h;
+    }
+  }
+}
+
+struct SweUpdateArgs {
+  double* x;
+  int64_t xes, xcs;
+  double* zeta;
+  double* sigma;
+  double* ha;
+  const double* k[4];
+  const double* zk[4];
+  const double* sk[4];
+  const double* hk[4];
+  double* surf;
+  double* face_h;
+  const double* mass;
+  const uint8_t* skip;
+  const uint8_t* mask;
+  int64_t n;
+  int is_face;
+};
+
+// SWERK4Update and the surface of the new state.
+__global__ __launch_bounds__(256) void swe_update_kernel(SweUpdateArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.skip && A.skip[i]) return;
+  constexpr double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
+  const int64_t n = A.n;
+  double* x = A.x + i * A.xes;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = i + c * n;
+    x[c * A.xcs] += sixth * (A.k[0][o] + A.k[3][o]) +
+                    third * (A.k[1][o] + A.k[2][o]);
+  }
+  A.zeta[i] += sixth * (A.zk[0][i] + A.zk[3][i]) + third * (A.zk[1][i] + A.zk[2][i]);
+  A.sigma[i] += sixth * (A.sk[0][i] + A.sk[3][i]) + third * (A.sk[1][i] + A.sk[2][i]);
+  const double h = A.ha[i] + (sixth * (A.hk[0][i] + A.hk[3][i]) +
+                              third * (A.hk[1][i] + A.hk[2][i]));
+  A.ha[i] = h;
+  if (!A.is_face) {
+    A.surf[i] = h;
+  } else if (!(A.mask && A.mask[i])) {
+    const double d = A.mass[i] / h;
+    A.face_h[i] = d;
+    A.surf[i] = d;
+  }
+}
+
+inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
+
+void validate(const lpm_swe_fields* f) {
+  LPM_ARG_CHECK(f, "null fields");
+  LPM_ARG_CHECK(f->n_verts >= 0 && f->n_faces > 0, "bad sizes");
+  if (f->n_verts > 0)
+    LPM_ARG_CHECK(f->vert_x && f->vert_zeta && f->vert_sigma && f->vert_h &&
+                      f->vert_u && f->vert_ddot && f->vert_laps && f->vert_surf &&
+                      f->vert_x_es > 0 && f->vert_x_cs > 0 && f->vert_u_es > 0 &&
+                      f->vert_u_cs > 0,
+                  "vertex fields incomplete");
+  LPM_ARG_CHECK(f->face_x && f->face_zeta && f->face_sigma && f->face_area &&
+                    f->face_mass && f->face_h && f->face_u && f->face_ddot &&
+                    f->face_laps && f->face_surf && f->face_x_es > 0 &&
+                    f->face_x_cs > 0 && f->face_u_es > 0 && f->face_u_cs > 0,
+                "face fields incomplete");
+}
+
+// Sums at positions (vx, fx) with face sources (fx, fzeta, fsigma, farea,
+// surf) into the fields' u, ddot, laps.
+void sums_eval(lpm_ctx* ctx, const lpm_swe_fields& f, const double* vx,
+               int64_t vxes, int64_t vxcs, const double* fx, int64_t fxes,
+               int64_t fxcs, const double* fzeta, const double* fsigma,
+               const double* farea, const RankMap& map, double eps,
+               double pse_eps, int precision, unsigned flags, hipStream_t s) {
+  SwePacked ps = swe_pack_sources(ctx, CrdIn{fx, fxes, fxcs}, fzeta, fsigma,
+                                  farea, f.face_surf, map, s);
+  SweTargets segs[2];
+  segs[0].base.x = CrdIn{vx, vxes, vxcs};
+  segs[0].base.n = f.n_verts;
+  segs[0].base.out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
+  segs[0].surf = f.vert_surf;
+  segs[0].ddot = f.vert_ddot;
+  segs[0].laps = f.vert_laps;
+  segs[1].base = face_targets(CrdIn{fx, fxes, fxcs}, map, f.face_mask,
+                              (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
+                              CrdOut{f.face_u, f.face_u_es, f.face_u_cs},
+                              nullptr);
+  segs[1].surf = f.face_surf;
+  segs[1].ddot = f.face_ddot;
+  segs[1].laps = f.face_laps;
+  launch_swe_sums(ctx, ps, segs, 2, eps, pse_eps, precision, s);
+}
+
+void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
+                    double Omega, double g, double eps, double pse_eps,
+                    int precision, unsigned flags, SweStage& V, SweStage& F,
+                    hipStream_t s) {
+  const uint8_t* fskip =
+      (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
+  const bool compat = (flags & LPM_RK4_REFERENCE_COMPAT) != 0;
+  const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+  const double coef[3] = {0.5, 0.5, 1.0};
+  for (int stage = 0; stage < 4; ++stage) {
+    if (stage > 0)
+      sums_eval(ctx, f, V.xw, 1, V.n, F.xw, 1, F.n, F.zw, F.sw, F.hw, map, eps,
+                pse_eps, precision, flags, s);
+    for (int kind = 0; kind < 2; ++kind) {
+      const bool face = kind == 1;
+      SweStage& B = face ? F : V;
+      const int64_t n = face ? f.n_faces : f.n_verts;
+      if (n == 0) continue;
+      SweStageArgs a{};
+      if (stage == 0) {
+        a.x = face ? f.face_x : f.vert_x;
+        a.xes = face ? f.face_x_es : f.vert_x_es;
+        a.xcs = face ? f.face_x_cs : f.vert_x_cs;
+        a.zeta = face ? f.face_zeta : f.vert_zeta;
+        a.sigma = face ? f.face_sigma : f.vert_sigma;
+        a.ha = face ? f.face_area : f.vert_h;
+      } else {
+        a.x = B.xw;
+        a.xes = 1;
+        a.xcs = B.n;
+        a.zeta = B.zw;
+        a.sigma = B.sw;
+        a.ha = B.hw;
+      }
+      a.u = face ? f.face_u : f.vert_u;
+      a.ues = face ? f.face_u_es : f.vert_u_es;
+      a.ucs = face ? f.face_u_cs : f.vert_u_cs;
+      a.ddot = face ? f.face_ddot : f.vert_ddot;
+      a.laps = face ? f.face_laps : f.vert_laps;
+      a.x0 = face ? f.face_x : f.vert_x;
+      a.x0es = face ? f.face_x_es : f.vert_x_es;
+      a.x0cs = face ? f.face_x_cs : f.vert_x_cs;
+      a.zeta0 = face ? f.face_zeta : f.vert_zeta;
+      a.sigma0 = face ? f.face_sigma : f.vert_sigma;
+      a.ha0 = face ? f.face_area : f.vert_h;
+      a.k = B.k[stage];
+      a.zk = B.zk[stage];
+      a.sk = B.sk[stage];
+      a.hk = B.hk[stage];
+      if (stage < 3) {
+        a.xw = B.xw;
+        a.zw = B.zw;
+        a.sw = B.sw;
+        a.hw = B.hw;
+        a.a = coef[stage];
+      }
+      a.surf = face ? f.face_surf : f.vert_surf;
+      a.face_h = f.face_h;
+      a.mass = f.face_mass;
+      a.skip = face ? fskip : nullptr;
+      a.mask = face ? f.face_mask : nullptr;
+      a.n = n;
+      a.dt = dt;
+      a.Omega = Omega;
+      a.g = g;
+      a.is_face = face ? 1 : 0;
+      // the reference never assigns the stage-4 position increment
+      a.store_k = (compat && stage == 3) ? 0 : 1;
+      swe_stage_kernel<<<nblk(n), 256, 0, s>>>(a);
+      LPM_HIP_CHECK(hipGetLastError());
+    }
+  }
+  for (int kind = 0; kind < 2; ++kind) {
+    const bool face = kind == 1;
+    SweStage& B = face ? F : V;
+    const int64_t n = face ? f.n_faces : f.n_verts;
+    if (n == 0) continue;
+    SweUpdateArgs u{};
+    u.x = face ? f.face_x : f.vert_x;
+    u.xes = face ? f.face_x_es : f.vert_x_es;
+    u.xcs = face ? f.face_x_cs : f.vert_x_cs;
+    u.zeta = face ? f.face_zeta : f.vert_zeta;
+    u.sigma = face ? f.face_sigma : f.vert_sigma;
+    u.ha = face ? f.face_area : f.vert_h;
+    for (int k = 0; k < 4; ++k) {
+      u.k[k] = B.k[k];
+      u.zk[k] = B.zk[k];
+      u.sk[k] = B.sk[k];
+      u.hk[k] = B.hk[k];
+    }
+    u.surf = face ? f.face_surf : f.vert_surf;
+    u.face_h = f.face_h;
+    u.mass = f.face_mass;
+    u.skip = face ? fskip : nullptr;
+    u.mask = face ? f.face_mask : nullptr;
+    u.n = n;
+    u.is_face = face ? 1 : 0;
+    swe_update_kernel<<<nblk(n), 256, 0, s>>>(u);
+    LPM_HIP_CHECK(hipGetLastError());
+  }
+  sums_eval(ctx, f, f.vert_x, f.vert_x_es, f.vert_x_cs, f.face_x, f.face_x_es,
+            f.face_x_cs, f.face_zeta, f.face_sigma, f.face_area, map, eps,
+            pse_eps, precision, flags, s);
+}
+
+template <typename F>
+int guarded(lpm_ctx* ctx, F&& fn) {
+  try {
+    fn();
+    return LPM_OK;
+  } catch (const Error& e) {
+    set_error(ctx, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(ctx, e.what());
+    return LPM_ERR_INTERNAL;
+  }
+}
+
+}  // namespace
+
+}  // namespace lpm_amd
+
+using namespace lpm_amd;
+
+extern "C" {
+
+int lpm_swe_sphere_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                        int64_t tgt_cs, int64_t n_tgt, const double* tgt_surf,
+                        const double* src_x, int64_t src_es, int64_t src_cs,
+                        const double* src_zeta, const double* src_sigma,
+                        const double* src_area, const uint8_t* src_mask,
+                        const double* src_surf, int64_t n_src, int collocated,
+                        double eps, double pse_eps, int precision,
+                        unsigned flags, double* out_u, int64_t out_es,
+                        int64_t out_cs, double* out_ddot, double* out_laps,
+                        void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(n_tgt >= 0 && n_src >= 0, "negative size");
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
+    if (n_tgt > 0) {
+      LPM_ARG_CHECK(tgt_x && tgt_es > 0 && tgt_cs > 0, "bad tgt_x");
+      LPM_ARG_CHECK(!out_u || (out_es > 0 && out_cs > 0), "bad out_u strides");
+    }
+    if (n_src > 0) {
+      LPM_ARG_CHECK(src_x && src_es > 0 && src_cs > 0, "bad src_x");
+      LPM_ARG_CHECK(src_zeta && src_sigma && src_area,
+                    "src_zeta/src_sigma/src_area are NULL");
+    }
+    if (pse_eps > 0)
+      LPM_ARG_CHECK((tgt_surf || n_tgt == 0) && (src_surf || n_src == 0),
+                    "PSE needs tgt_surf and src_surf");
+    if (collocated)
+      LPM_ARG_CHECK(n_tgt == n_src,
+                    "collocated: targets must be the sources (n_tgt == n_src)");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, src_mask, n_src, s);
+    SwePacked ps = swe_pack_sources(ctx, CrdIn{src_x, src_es, src_cs},
+                                    src_zeta, src_sigma, src_area, src_surf,
+                                    map, s);
+    SweTargets t;
+    if (collocated) {
+      t.base = face_targets(CrdIn{tgt_x, tgt_es, tgt_cs}, map, src_mask,
+                            (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
+                            CrdOut{out_u, out_es, out_cs}, nullptr);
+    } else {
+      t.base.x = CrdIn{tgt_x, tgt_es, tgt_cs};
+      t.base.n = n_tgt;
+      t.base.out = CrdOut{out_u, out_es, out_cs};
+    }
+    t.surf = tgt_surf;
+    t.ddot = out_ddot;
+    t.laps = out_laps;
+    launch_swe_sums(ctx, ps, &t, 1, eps, pse_eps, precision, s);
+    if ((flags & LPM_FLAG_CHECK_FINITE) && out_u)
+      check_finite(ctx, out_u, n_tgt, out_es, out_cs, 3, s);
+  });
+}
+
+int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
+                      double pse_eps, int precision, unsigned flags,
+                      void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    sums_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
+              f->face_x_es, f->face_x_cs, f->face_zeta, f->face_sigma,
+              f->face_area, map, eps, pse_eps, precision, flags, s);
+  });
+}
+
+int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
+                     double Omega, double g, double eps, double pse_eps,
+                     int precision, unsigned flags, void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega) && std::isfinite(g),
+                  "dt, Omega, g must be finite");
+    LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
+    LPM_ARG_CHECK(!(flags & LPM_RK4_USE_GRAPH),
+                  "graph capture is available for the BVE stepper only");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    SweStage V = carve(ctx->swe_v, f->n_verts);
+    SweStage F = carve(ctx->swe_f, f->n_faces);
+    issue_swe_step(ctx, *f, dt, Omega, g, eps, pse_eps, precision, flags, V, F,
+                   s);
+    if (flags & LPM_FLAG_CHECK_FINITE) {
+      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
+      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
+    }
+  });
+}
+
+}  // extern "C"

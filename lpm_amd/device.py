@@ -117,6 +117,70 @@ class Context:
             precision_id(precision), flags,
             stream if stream is not None else current_stream()))
 
+    # ----------------------------------------------------------- SWE sums
+    def swe_sums(self, tgt_x, tgt_surf, src_x, src_zeta, src_sigma, src_area,
+                 src_surf, src_mask=None, collocated=False, eps=0.0,
+                 pse_eps=0.0, precision="fp64", flags=0, stream=None):
+        """Returns (u (3, n), ddot (n), laps (n))."""
+        tp, tes, tcs, nt = crd_args(tgt_x)
+        sp, ses, scs, ns = crd_args(src_x)
+        dev = tgt_x.device
+        u = torch.zeros((3, nt), dtype=torch.float64, device=dev)
+        dd = torch.zeros(nt, dtype=torch.float64, device=dev)
+        lap = torch.zeros(nt, dtype=torch.float64, device=dev)
+        op, oes, ocs, _ = crd_args(u)
+        self.check(self.L.lpm_swe_sphere_sums(
+            self.h, tp, tes, tcs, nt, _ptr(tgt_surf), sp, ses, scs,
+            _ptr(src_zeta), _ptr(src_sigma), _ptr(src_area), _ptr(src_mask),
+            _ptr(src_surf), ns, int(collocated), float(eps), float(pse_eps),
+            precision_id(precision), flags, op, oes, ocs, _ptr(dd), _ptr(lap),
+            stream if stream is not None else current_stream()))
+        return u, dd, lap
+
+    def swe_init_sums(self, fields, eps, pse_eps, precision="fp64", flags=0,
+                      stream=None):
+        self.check(self.L.lpm_swe_init_sums(
+            self.h, ctypes.byref(fields), float(eps), float(pse_eps),
+            precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
+    def swe_rk4_step(self, fields, dt, Omega, g, eps, pse_eps,
+                     precision="fp64", flags=0, stream=None):
+        self.check(self.L.lpm_swe_rk4_step(
+            self.h, ctypes.byref(fields), float(dt), float(Omega), float(g),
+            float(eps), float(pse_eps), precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
+    # ----------------------------------------------------------- variants
+    def variants(self, kernel_id):
+        L = self.L
+        return [L.lpm_variant_name(kernel_id, v).decode()
+                for v in range(L.lpm_variant_count(kernel_id))]
+
+    def set_variant(self, kernel_id, variant):
+        self.check(self.L.lpm_set_variant(self.h, kernel_id, variant))
+
+    def bve_autotune(self, fields, precision="fp64", flags=0, repeats=2,
+                     stream=None):
+        """Returns (best variant, its ms, list of ms of every variant)."""
+        kid = (_abi.LPM_KERNEL_BVE_VEL_F64 if precision_id(precision) == _abi.LPM_FP64
+               else _abi.LPM_KERNEL_BVE_VEL_MIXED)
+        nv = self.L.lpm_variant_count(kid)
+        best = ctypes.c_int()
+        ms = ctypes.c_double()
+        all_ms = (ctypes.c_double * nv)()
+        self.check(self.L.lpm_bve_autotune(
+            self.h, ctypes.byref(fields), precision_id(precision), flags,
+            repeats, ctypes.byref(best), ctypes.byref(ms), all_ms,
+            stream if stream is not None else current_stream()))
+        return best.value, ms.value, list(all_ms)
+
+    def tune_save(self, path):
+        self.check(self.L.lpm_tune_save(self.h, path.encode()))
+
+    def tune_load(self, path):
+        self.check(self.L.lpm_tune_load(self.h, path.encode()))
+
     # ------------------------------------------------------------- timing
     def timing_enable(self, enable=True):
         self.check(self.L.lpm_timing_enable(self.h, int(enable)))

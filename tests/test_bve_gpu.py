@@ -141,7 +141,9 @@ def test_skip_masked_targets_leaves_them_untouched(ctx):
                             dev(m.face_mask, torch.uint8), collocated=True)
     div = torch.from_numpy(m.face_mask == 1)
     assert torch.all(out[:, div] == 7.0)
-    assert torch.equal(out[:, ~div], full[:, ~div])
+    # compacted targets are grouped into different waves, which may take the
+    # self-exclusion path for different tiles: equal to rounding
+    assert rel_l2(out[:, ~div].cpu().numpy(), full[:, ~div].cpu().numpy()) < 1e-14
 
 
 def test_edge_cases(ctx, ora):
@@ -155,10 +157,18 @@ def test_edge_cases(ctx, ora):
         aa = rng.uniform(0.1, 1.0, size=ns)
         for prec in ("fp64", "mixed"):
             u = ctx.bve_velocity(dev(x), dev(y), dev(zz), dev(aa), None,
-                                 precision=prec)
+                                 precision=prec).cpu().numpy().T
             ref = ora.bve_velocity(aos(x), aos(y), zz, aa)
-            tol = TOL64 if prec == "fp64" else 1e-5
-            assert rel_l2(u.cpu().numpy().T, ref) < tol, (ns, prec)
+            if prec == "fp64":
+                # random clouds hold close pairs where 1 - x.y cancels; the
+                # reference formula itself then strays from the exact sum,
+                # so judge against the extended-precision sum and allow the
+                # reference formula's own error when comparing to it
+                xp = ora.bve_velocity_xp(aos(x), aos(y), zz, aa)
+                assert rel_l2(u, xp) < TOL64, (ns, prec)
+                assert rel_l2(u, ref) < TOL64 + rel_l2(ref, xp), (ns, prec)
+            else:
+                assert rel_l2(u, ref) < 1e-5, (ns, prec)
     # no targets
     y = dev(np.eye(3))
     out = ctx.bve_velocity(torch.zeros((3, 0), dtype=torch.float64, device="cuda"),
@@ -275,6 +285,67 @@ def test_rk4_solid_body_rotation(ctx):
     ex = np.stack([c * x0[0] - sn * x0[1], sn * x0[0] + c * x0[1], x0[2]])
     err = np.max(np.linalg.norm(s.vert_x.cpu().numpy() - ex, axis=0))
     assert err < 1e-2 * ang
+
+
+@pytest.mark.parametrize("kid,prec", [(_abi.LPM_KERNEL_BVE_VEL_F64, "fp64"),
+                                      (_abi.LPM_KERNEL_BVE_VEL_MIXED, "mixed")])
+def test_every_variant_matches_oracle(ctx, ora, kid, prec):
+    m, z = _inputs(1, 3)
+    vx, fx = dev(m.vert_x), dev(m.face_x)
+    args = (dev(z), dev(m.face_area), dev(m.face_mask, torch.uint8))
+    ref_v = ora.bve_velocity(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask)
+    ref_f = ora.bve_velocity(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask, collocated=True)
+    tol = TOL64 if prec == "fp64" else TOL_MIXED
+    names = ctx.variants(kid)
+    assert len(names) >= 4
+    try:
+        for v in range(len(names)):
+            ctx.set_variant(kid, v)
+            for eps in (0.0, 0.05):
+                uv = ctx.bve_velocity(vx, fx, *args, precision=prec, eps=eps)
+                uf = ctx.bve_velocity(fx, fx, *args, collocated=True,
+                                      precision=prec, eps=eps)
+                if eps == 0.0:
+                    assert rel_l2(uv.cpu().numpy().T, ref_v) < tol, names[v]
+                    assert rel_l2(uf.cpu().numpy().T, ref_f) < tol, names[v]
+                else:
+                    ctx.set_variant(kid, 0)
+                    ev = ctx.bve_velocity(vx, fx, *args, precision=prec, eps=eps)
+                    ctx.set_variant(kid, v)
+                    assert rel_l2(uv.cpu().numpy(), ev.cpu().numpy()) < tol, names[v]
+    finally:
+        ctx.set_variant(kid, -1)
+
+
+def test_regularised_kernel_matches_pair_formula(ctx, ora):
+    # eps > 0: the pair term is kzeta_sphere with eps (the BVE kernel with
+    # 1 - x.y + eps^2 in the denominator)
+    m, z = _inputs(0, 2)
+    eps = 0.05
+    u = ctx.bve_velocity(dev(m.vert_x), dev(m.face_x), dev(z), dev(m.face_area),
+                         dev(m.face_mask, torch.uint8), eps=eps).cpu().numpy().T
+    vx, fx = aos(m.vert_x), aos(m.face_x)
+    ref = np.zeros_like(vx)
+    for j in np.where(m.face_mask == 0)[0]:
+        for i in range(len(vx)):
+            ref[i] += ora.kzeta_sphere(vx[i], fx[j], z[j], m.face_area[j], eps)
+    assert rel_l2(u, ref) < TOL64
+
+
+def test_autotune_records_a_variant(ctx, tmp_path):
+    s = _sphere(ctx, 0, 4)
+    for prec, kid in (("fp64", _abi.LPM_KERNEL_BVE_VEL_F64),
+                      ("mixed", _abi.LPM_KERNEL_BVE_VEL_MIXED)):
+        best, ms, all_ms = ctx.bve_autotune(
+            s.fields(), prec, flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS, repeats=1)
+        assert 0 <= best < len(ctx.variants(kid))
+        assert ms > 0 and ms == min(all_ms)
+    path = str(tmp_path / "tune.txt")
+    ctx.tune_save(path)
+    assert "gfx950" in open(path).read() or len(open(path).read()) > 0
+    ctx.tune_load(path)
 
 
 def test_timing_records_launches(ctx):
