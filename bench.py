@@ -17,12 +17,19 @@ CPU oracle's rate on a bounded sample of the same evaluation.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints exactly one JSON line on stdout; progress goes to stderr.
+
+``--copy-calibration MIB`` instead runs a plain device copy (read MIB, write
+MIB) a few times: the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of that run
+scale the counters of the direct-sum kernel to bytes.  The measured traffic
+per launch of the FP64 kernel is read from
+profiles/bve_vel_f64_icos_l7_traffic.md when that summary is present.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -142,6 +149,42 @@ def cpu_baseline(mesh, z_faces, budget_s=12.0):
     }, sample, stride, u
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
+                               "bve_vel_f64_icos_l7_traffic.md")
+
+
+def measured_traffic(level, variant):
+    """HBM bytes per launch of the FP64 kernel from the committed rocprofv3
+    counter summary (line ``traffic_bytes_per_launch: <variant> <level>
+    <bytes>``), or None when it does not cover this variant and level."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+    except OSError:
+        return None
+    for m in re.finditer(r"^traffic_bytes_per_launch:\s+(\S+)\s+(\d+)\s+(\d+)",
+                         text, re.M):
+        if m.group(1) == variant and int(m.group(2)) == level:
+            return int(m.group(3))
+    return None
+
+
+def copy_calibration(mib, reps=5):
+    """A plain elementwise copy of known size for scaling byte counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {
+        "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+        "launches": reps, "kernel": "torch.neg (elementwise, f64)"}}),
+        flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +199,11 @@ def main(argv=None):
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the fixed default kernel variants")
+    ap.add_argument("--variant-f64", default=None,
+                    help="force this FP64 kernel variant (implies no autotune "
+                         "of the FP64 run)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -164,6 +212,8 @@ def main(argv=None):
               "with an all-gather of sources per RK stage), not implemented",
               file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
 
     import numpy as np
     import torch
@@ -179,10 +229,20 @@ def main(argv=None):
     ctx = Context(0)
     props = torch.cuda.get_device_properties(0)
 
+    from lpm_amd import _abi
+    kid64 = _abi.LPM_KERNEL_BVE_VEL_F64
+    names64 = ctx.variants(kid64)
+    if a.variant_f64 is not None:
+        if a.variant_f64 not in names64:
+            log(f"unknown FP64 variant {a.variant_f64}; have {names64}")
+            return 2
+        ctx.set_variant(kid64, names64.index(a.variant_f64))
     log("FP64 run")
     r64 = run_precision(ctx, mesh, "fp64", a.steps, a.warmup, a.dt,
                         dict(reference_compat=True),
-                        autotune=not a.no_autotune)
+                        autotune=not a.no_autotune and a.variant_f64 is None)
+    v64 = (r64["autotune"] or {}).get("variant") or (
+        a.variant_f64 or names64[ctx.L.lpm_variant_default(kid64)])
     step_s = r64["elapsed_s"] / a.steps
     value = 4 * pairs_eval * a.steps / r64["elapsed_s"] / 1e9
     avg_ms = r64["kernel_ms_total"] / max(r64["launches"], 1)
@@ -229,9 +289,9 @@ def main(argv=None):
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(a.level, v64),
             "kernel": "bve_vel_f64_kernel",
-            "variant": (r64["autotune"] or {}).get("variant"),
+            "variant": v64,
             "avg_launch_ms": round(avg_ms, 4),
             "flops_per_pair": F_BVE,
             "pairs_per_launch": pairs_eval,

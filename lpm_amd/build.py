@@ -36,6 +36,7 @@ HIP_FLAGS = [
 # host-only sources: strict IEEE operation order for the mesh generator
 HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
 HIP_SOURCES = ["bve_sums.hip", "bve_rk4.hip", "swe_sums.hip", "swe_rk4.hip"]
+EXAMPLES = ["bve_rotation", "sphere_swe_tc2"]
 
 
 def _sources():
@@ -82,7 +83,24 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    for ex in EXAMPLES:
+        _build_example(ex, verbose)
     return LIB
+
+
+def _build_example(name, verbose=False):
+    """C++ drivers over the C++ API (host code only), linked to the library."""
+    src = os.path.join(HERE, "examples", name + ".cpp")
+    out = os.path.join(BUILD, name)
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(HERE, 'cpp', 'include')}", src, "-o", out,
+           f"-L{HERE}", "-llpm_mi355x", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build of example {name} failed:\n{r.stderr}")
+    return out
 
 
 def main(argv=None):

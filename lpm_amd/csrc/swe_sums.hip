@@ -107,9 +107,11 @@ __device__ __forceinline__ double rcp_nr(double d) {
 // x.y > 0 evaluated with the reference's rounding, ((x0 y0 + x1 y1) + x2 y2)
 // without contraction, so that points on the cut (x.y = 0 in exact
 // arithmetic, common on the cubed sphere) are classified identically.
+// __dmul_rn / __dadd_rn are plain * and + in this toolchain and would still
+// be fused into FMAs; the pragma keeps three multiplies and two adds.
 __device__ __forceinline__ bool hemisphere(const double* x, const double4& y) {
-  const double t = __dadd_rn(__dadd_rn(__dmul_rn(x[0], y.x), __dmul_rn(x[1], y.y)),
-                             __dmul_rn(x[2], y.z));
+#pragma clang fp contract(off)
+  const double t = (x[0] * y.x + x[1] * y.y) + x[2] * y.z;
   return t > 0.0;
 }
 
