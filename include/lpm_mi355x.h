@@ -184,6 +184,44 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                      double Omega, int precision, unsigned flags,
                      void* stream);
 
+/* -------------------------------------------------------- ensembles ---- */
+/* An ensemble of independent BVE problems ("members") stored back to back
+ * in one lpm_bve_fields: member m owns vertices [vert_offsets[m],
+ * vert_offsets[m + 1]) and faces [face_offsets[m], face_offsets[m + 1]) of
+ * the concatenated arrays (strides apply to the concatenated arrays), and
+ * its velocity is the sum over its own undivided faces only.  The offset
+ * arrays are host memory of n_members + 1 nondecreasing entries from 0 to
+ * n_verts / n_faces; every member owns at least one face.
+ * All members are evaluated by one persistent kernel that takes
+ * (member, target block) work items from a queue, largest members first,
+ * so members of very different sizes share one launch.  The work list is
+ * uploaded (synchronously) the first time an offset layout is seen and
+ * cached in the context.
+ * No reference counterpart: the reference evolves one BVESphere at a time
+ * (BVESphere::init_velocity, src/lpm_bve_sphere_impl.hpp:181-205;
+ * BVERK4::advance_timestep, src/lpm_bve_rk4_impl.hpp:55-167).  Each
+ * member's result equals lpm_bve_init_velocity / lpm_bve_rk4_step applied
+ * to that member alone, bit for bit. */
+typedef struct lpm_bve_ensemble {
+  int32_t n_members;
+  const int64_t* vert_offsets; /* host, n_members + 1 */
+  const int64_t* face_offsets; /* host, n_members + 1 */
+  lpm_bve_fields fields;       /* concatenated member arrays */
+} lpm_bve_ensemble;
+
+/* Velocity of every member at its current state (the ensemble form of
+ * lpm_bve_init_velocity; same flags). */
+int lpm_bve_ensemble_init_velocity(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                                   int precision, unsigned flags,
+                                   void* stream);
+
+/* One RK4 step of every member (the ensemble form of lpm_bve_rk4_step;
+ * same flags, including LPM_RK4_USE_GRAPH, which replays the whole step,
+ * with its 4 ensemble launches, from one hipGraph). */
+int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                              double dt, double Omega, int precision,
+                              unsigned flags, void* stream);
+
 /* --------------------------------------------------------- SWE sums ---- */
 /* Sphere shallow-water direct sums at the targets: velocity from relative
  * vorticity zeta and divergence sigma,
@@ -280,6 +318,12 @@ int lpm_set_variant(lpm_ctx* ctx, int kernel_id, int variant);
 int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
                      unsigned flags, int repeats, int* best_variant,
                      double* best_ms, double* all_ms_host, void* stream);
+/* The same for the ensemble kernel (LPM_KERNEL_BVE_ENSEMBLE_*), keyed by
+ * the ensemble's total target count. */
+int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                              int precision, unsigned flags, int repeats,
+                              int* best_variant, double* best_ms,
+                              double* all_ms_host, void* stream);
 /* Tuning table persistence (text: "arch/kernel/bucket variant" lines). */
 int lpm_tune_save(lpm_ctx* ctx, const char* path);
 int lpm_tune_load(lpm_ctx* ctx, const char* path);
@@ -296,7 +340,8 @@ int lpm_tune_load(lpm_ctx* ctx, const char* path);
 #define LPM_KERNEL_BVE_PSI 2
 #define LPM_KERNEL_SWE_F64 3
 #define LPM_KERNEL_SWE_MIXED 4
-#define LPM_KERNEL_BVE_BATCHED 5
+#define LPM_KERNEL_BVE_ENSEMBLE_F64 5
+#define LPM_KERNEL_BVE_ENSEMBLE_MIXED 6
 int lpm_timing_enable(lpm_ctx* ctx, int enable);
 int lpm_timing_query(lpm_ctx* ctx, int kernel_id, double* total_ms,
                      int64_t* launches, double* pairs);

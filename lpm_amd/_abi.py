@@ -36,7 +36,8 @@ LPM_KERNEL_BVE_VEL_MIXED = 1
 LPM_KERNEL_BVE_PSI = 2
 LPM_KERNEL_SWE_F64 = 3
 LPM_KERNEL_SWE_MIXED = 4
-LPM_KERNEL_BVE_BATCHED = 5
+LPM_KERNEL_BVE_ENSEMBLE_F64 = 5
+LPM_KERNEL_BVE_ENSEMBLE_MIXED = 6
 
 c_int = ctypes.c_int
 c_uint = ctypes.c_uint
@@ -66,6 +67,14 @@ class BveFields(ctypes.Structure):
         ("face_x_cs", c_i64), ("face_zeta", c_vp), ("face_u", c_vp),
         ("face_u_es", c_i64), ("face_u_cs", c_i64), ("face_area", c_vp),
         ("face_mask", c_vp),
+    ]
+
+
+class BveEnsemble(ctypes.Structure):
+    """struct lpm_bve_ensemble (offsets are host int64 arrays)."""
+    _fields_ = [
+        ("n_members", ctypes.c_int32), ("vert_offsets", c_i64p),
+        ("face_offsets", c_i64p), ("fields", BveFields),
     ]
 
 
@@ -110,6 +119,13 @@ _SIGS = {
                                       c_uint, c_vp]),
     "lpm_bve_rk4_step": (c_int, [c_vp, ctypes.POINTER(BveFields), c_dbl, c_dbl,
                                  c_int, c_uint, c_vp]),
+    "lpm_bve_ensemble_init_velocity": (c_int, [c_vp, ctypes.POINTER(BveEnsemble),
+                                               c_int, c_uint, c_vp]),
+    "lpm_bve_ensemble_rk4_step": (c_int, [c_vp, ctypes.POINTER(BveEnsemble),
+                                          c_dbl, c_dbl, c_int, c_uint, c_vp]),
+    "lpm_bve_ensemble_autotune": (c_int, [c_vp, ctypes.POINTER(BveEnsemble),
+                                          c_int, c_uint, c_int, c_intp, c_dblp,
+                                          c_dblp, c_vp]),
     "lpm_swe_sphere_sums": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp,
                                     c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                                     c_i64, c_int, c_dbl, c_dbl, c_int, c_uint,

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <sstream>
 
 #include "internal.hpp"
@@ -139,43 +140,36 @@ __global__ __launch_bounds__(256) void rk_update_kernel(UpdateArgs A) {
 
 inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
-void velocity_eval(lpm_ctx* ctx, const lpm_bve_fields& f, const double* vx,
-                   int64_t vxes, int64_t vxcs, const double* fx, int64_t fxes,
-                   int64_t fxcs, const double* fzeta, const RankMap& map,
-                   int precision, unsigned flags, hipStream_t s) {
-  PackedSources ps = pack_sources(ctx, CrdIn{fx, fxes, fxcs}, fzeta,
-                                  f.face_area, map, precision, s);
+// Velocity at vertex positions vx and face positions fx with face vorticity
+// fzeta, written to f.vert_u / f.face_u.
+using EvalFn =
+    std::function<void(const RankMap&, CrdIn vx, CrdIn fx, const double* fz)>;
+
+void velocity_eval(lpm_ctx* ctx, const lpm_bve_fields& f, CrdIn vx, CrdIn fx,
+                   const double* fzeta, const RankMap& map, int precision,
+                   unsigned flags, hipStream_t s) {
+  PackedSources ps =
+      pack_sources(ctx, fx, fzeta, f.face_area, map, precision, s);
   TargetSeg segs[2];
-  segs[0].x = CrdIn{vx, vxes, vxcs};
+  segs[0].x = vx;
   segs[0].n = f.n_verts;
   segs[0].out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
-  segs[1] = face_targets(CrdIn{fx, fxes, fxcs}, map, f.face_mask,
+  segs[1] = face_targets(fx, map, f.face_mask,
                          (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
                          CrdOut{f.face_u, f.face_u_es, f.face_u_cs}, nullptr);
   launch_bve_velocity(ctx, ps, segs, 2, 0.0, precision, s);
 }
 
-void validate(const lpm_bve_fields* f) {
-  LPM_ARG_CHECK(f, "null fields");
-  LPM_ARG_CHECK(f->n_verts >= 0 && f->n_faces >= 0, "negative sizes");
-  if (f->n_verts > 0) {
-    LPM_ARG_CHECK(f->vert_x && f->vert_zeta && f->vert_u,
-                  "vertex fields are NULL");
-    LPM_ARG_CHECK(f->vert_x_es > 0 && f->vert_x_cs > 0 && f->vert_u_es > 0 &&
-                      f->vert_u_cs > 0,
-                  "vertex strides must be > 0");
-  }
-  LPM_ARG_CHECK(f->n_faces > 0, "at least one face is required");
-  LPM_ARG_CHECK(f->face_x && f->face_zeta && f->face_u && f->face_area,
-                "face fields are NULL");
-  LPM_ARG_CHECK(f->face_x_es > 0 && f->face_x_cs > 0 && f->face_u_es > 0 &&
-                    f->face_u_cs > 0,
-                "face strides must be > 0");
+CrdIn vert_crd(const lpm_bve_fields& f) {
+  return CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs};
+}
+CrdIn face_crd(const lpm_bve_fields& f) {
+  return CrdIn{f.face_x, f.face_x_es, f.face_x_cs};
 }
 
 void issue_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt,
-                double Omega, int precision, unsigned flags, StageBufs& V,
-                StageBufs& F, hipStream_t s) {
+                double Omega, unsigned flags, StageBufs& V, StageBufs& F,
+                const EvalFn& eval, hipStream_t s) {
   const int renorm = (flags & LPM_RK4_RENORMALIZE) ? 1 : 0;
   const uint8_t* fskip =
       (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
@@ -184,8 +178,7 @@ void issue_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt,
 
   for (int stage = 0; stage < 4; ++stage) {
     if (stage > 0)
-      velocity_eval(ctx, f, V.xw, 1, V.n, F.xw, 1, F.n, F.zw, map, precision,
-                    flags, s);
+      eval(map, CrdIn{V.xw, 1, V.n}, CrdIn{F.xw, 1, F.n}, F.zw);
     StageArgs a{};
     a.dt = dt;
     a.Omega = Omega;
@@ -256,12 +249,12 @@ void issue_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt,
   rk_update_kernel<<<nblk(u.n), 256, 0, s>>>(u);
   LPM_HIP_CHECK(hipGetLastError());
 
-  velocity_eval(ctx, f, f.vert_x, f.vert_x_es, f.vert_x_cs, f.face_x,
-                f.face_x_es, f.face_x_cs, f.face_zeta, map, precision, flags, s);
+  eval(map, vert_crd(f), face_crd(f), f.face_zeta);
 }
 
 std::string graph_key(const lpm_bve_fields& f, double dt, double Omega,
-                      int precision, unsigned flags, hipStream_t s) {
+                      int precision, unsigned flags, hipStream_t s,
+                      uint64_t plan_serial) {
   std::ostringstream k;
   k << "bve|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
     << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_u << '|'
@@ -271,78 +264,98 @@ std::string graph_key(const lpm_bve_fields& f, double dt, double Omega,
     << f.face_u_cs << '|' << f.face_area << '|'
     << static_cast<const void*>(f.face_mask) << '|';
   k.precision(17);
-  k << dt << '|' << Omega << '|' << precision << '|' << flags << '|' << s;
+  k << dt << '|' << Omega << '|' << precision << '|' << flags << '|' << s
+    << '|' << plan_serial;
   return k.str();
 }
 
-}  // namespace
-
-}  // namespace lpm_amd
-
-using namespace lpm_amd;
-
-extern "C" {
-
-int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
-                          int precision, unsigned flags, void* stream) {
-  try {
-    LPM_ARG_CHECK(ctx, "null context");
-    validate(f);
-    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
-                  "unknown precision");
-    LPM_HIP_CHECK(hipSetDevice(ctx->device));
-    hipStream_t s = as_stream(stream);
-    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
-    velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
-                  f->face_x_es, f->face_x_cs, f->face_zeta, map, precision,
-                  flags, s);
-    if (flags & LPM_FLAG_CHECK_FINITE) {
-      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
-      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
-    }
-    return LPM_OK;
-  } catch (const Error& e) {
-    set_error(ctx, e.what());
-    return e.code;
-  } catch (const std::exception& e) {
-    set_error(ctx, e.what());
-    return LPM_ERR_INTERNAL;
+// One RK4 step, directly or through a captured graph.  prepare() allocates
+// every workspace the step uses and returns a serial that changes whenever
+// the launches it would record change (0 if nothing beyond the fields).
+void run_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt, double Omega,
+              int precision, unsigned flags, hipStream_t s,
+              const std::function<uint64_t()>& prepare, const EvalFn& eval) {
+  StageBufs V = carve(ctx->rk_v, f.n_verts);
+  StageBufs F = carve(ctx->rk_f, f.n_faces);
+  if (!(flags & LPM_RK4_USE_GRAPH)) {
+    issue_step(ctx, f, dt, Omega, flags, V, F, eval, s);
+    return;
   }
+  LPM_ARG_CHECK(s != nullptr, "LPM_RK4_USE_GRAPH needs a non-default stream");
+  // no allocation may happen while the stream is being captured
+  const uint64_t serial = prepare();
+  LPM_HIP_CHECK(hipStreamSynchronize(s));
+  const std::string key =
+      graph_key(f, dt, Omega, precision, flags, s, serial);
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end() &&
+      it->second->epoch != g_workspace_epoch.load()) {
+    ctx->graphs.erase(it);
+    it = ctx->graphs.end();
+  }
+  if (it == ctx->graphs.end()) {
+    auto entry = std::make_shared<lpm_ctx::GraphEntry>();
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    LPM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      issue_step(ctx, f, dt, Omega, flags, V, F, eval, s);
+    } catch (...) {
+      ctx->timing = timing;
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    ctx->timing = timing;
+    LPM_HIP_CHECK(hipStreamEndCapture(s, &entry->graph));
+    LPM_HIP_CHECK(
+        hipGraphInstantiate(&entry->exec, entry->graph, nullptr, nullptr, 0));
+    entry->epoch = g_workspace_epoch.load();
+    it = ctx->graphs.emplace(key, entry).first;
+  }
+  LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
 }
 
-int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
-                     unsigned flags, int repeats, int* best_variant,
-                     double* best_ms, double* all_ms, void* stream) {
+void check_outputs(lpm_ctx* ctx, const lpm_bve_fields& f, unsigned flags,
+                   hipStream_t s) {
+  if (!(flags & LPM_FLAG_CHECK_FINITE)) return;
+  check_finite(ctx, f.vert_u, f.n_verts, f.vert_u_es, f.vert_u_cs, 3, s);
+  check_finite(ctx, f.face_u, f.n_faces, f.face_u_es, f.face_u_cs, 3, s);
+}
+
+int fail(lpm_ctx* ctx, const std::exception& e) {
+  set_error(ctx, e.what());
+  const Error* le = dynamic_cast<const Error*>(&e);
+  return le ? le->code : LPM_ERR_INTERNAL;
+}
+
+// Times every variant of `family` on eval() and records the fastest under
+// the tuning key of n_targets.
+void autotune(lpm_ctx* ctx, int family, int64_t n_targets, int repeats,
+              const std::function<void()>& eval, hipStream_t s, int* best_variant,
+              double* best_ms, double* all_ms) {
+  LPM_ARG_CHECK(repeats >= 1 && repeats <= 100, "repeats must be in [1, 100]");
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int family = precision == LPM_MIXED ? KID_BVE_VEL_MIXED : KID_BVE_VEL_F64;
-  int saved = -1;
-  bool timing = false;
+  const int saved = ctx->forced[family];
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  auto restore = [&]() {
+    ctx->forced[family] = saved;
+    ctx->timing = timing;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
   try {
-    LPM_ARG_CHECK(ctx, "null context");
-    validate(f);
-    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
-                  "unknown precision");
-    LPM_ARG_CHECK(repeats >= 1 && repeats <= 100, "repeats must be in [1, 100]");
-    LPM_HIP_CHECK(hipSetDevice(ctx->device));
-    hipStream_t s = as_stream(stream);
-    saved = ctx->forced[family];
-    timing = ctx->timing;
-    ctx->timing = false;
     LPM_HIP_CHECK(hipEventCreate(&e0));
     LPM_HIP_CHECK(hipEventCreate(&e1));
-    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
     int best = -1;
     float best_t = 0;
     for (int v = 0; v < variant_count(family); ++v) {
       ctx->forced[family] = v;
-      velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
-                    f->face_x_es, f->face_x_cs, f->face_zeta, map, precision,
-                    flags, s);
+      eval();  // warm-up (and plan/workspace set-up for this variant)
       LPM_HIP_CHECK(hipEventRecord(e0, s));
-      for (int r = 0; r < repeats; ++r)
-        velocity_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs,
-                      f->face_x, f->face_x_es, f->face_x_cs, f->face_zeta, map,
-                      precision, flags, s);
+      for (int r = 0; r < repeats; ++r) eval();
       LPM_HIP_CHECK(hipEventRecord(e1, s));
       LPM_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0;
@@ -354,105 +367,205 @@ int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
         best_t = ms;
       }
     }
-    ctx->forced[family] = saved;
-    ctx->timing = timing;
-    ctx->tuned[tune_key(ctx, family, f->n_verts + f->n_faces)] = best;
+    restore();
+    ctx->tuned[tune_key(ctx, family, n_targets)] = best;
     if (best_variant) *best_variant = best;
     if (best_ms) *best_ms = best_t;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+  } catch (...) {
+    restore();
+    throw;
+  }
+}
+
+}  // namespace
+
+void validate_bve_fields(const lpm_bve_fields* f) {
+  LPM_ARG_CHECK(f, "null fields");
+  LPM_ARG_CHECK(f->n_verts >= 0 && f->n_faces >= 0, "negative sizes");
+  if (f->n_verts > 0) {
+    LPM_ARG_CHECK(f->vert_x && f->vert_zeta && f->vert_u,
+                  "vertex fields are NULL");
+    LPM_ARG_CHECK(f->vert_x_es > 0 && f->vert_x_cs > 0 && f->vert_u_es > 0 &&
+                      f->vert_u_cs > 0,
+                  "vertex strides must be > 0");
+  }
+  LPM_ARG_CHECK(f->n_faces > 0, "at least one face is required");
+  LPM_ARG_CHECK(f->face_x && f->face_zeta && f->face_u && f->face_area,
+                "face fields are NULL");
+  LPM_ARG_CHECK(f->face_x_es > 0 && f->face_x_cs > 0 && f->face_u_es > 0 &&
+                    f->face_u_cs > 0,
+                "face strides must be > 0");
+}
+
+}  // namespace lpm_amd
+
+using namespace lpm_amd;
+
+namespace {
+
+void check_precision(int precision) {
+  LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                "unknown precision");
+}
+
+EvalFn single_eval(lpm_ctx* ctx, const lpm_bve_fields& f, int precision,
+                   unsigned flags, hipStream_t s) {
+  return [ctx, &f, precision, flags, s](const RankMap& map, CrdIn vx, CrdIn fx,
+                                        const double* fz) {
+    velocity_eval(ctx, f, vx, fx, fz, map, precision, flags, s);
+  };
+}
+
+EvalFn ensemble_eval(lpm_ctx* ctx, const EnsembleDesc& d, int precision,
+                     unsigned flags, hipStream_t s) {
+  return [ctx, &d, precision, flags, s](const RankMap& map, CrdIn vx, CrdIn fx,
+                                        const double* fz) {
+    ensemble_velocity(ctx, d, vx, fx, fz, map, precision, flags, s);
+  };
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
+                          int precision, unsigned flags, void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate_bve_fields(f);
+    check_precision(precision);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    single_eval(ctx, *f, precision, flags, s)(map, vert_crd(*f), face_crd(*f),
+                                              f->face_zeta);
+    check_outputs(ctx, *f, flags, s);
     return LPM_OK;
   } catch (const std::exception& e) {
-    if (ctx) {
-      ctx->forced[family] = saved;
-      ctx->timing = timing;
-      set_error(ctx, e.what());
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    const Error* le = dynamic_cast<const Error*>(&e);
-    return le ? le->code : LPM_ERR_INTERNAL;
+    return fail(ctx, e);
+  }
+}
+
+int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
+                     unsigned flags, int repeats, int* best_variant,
+                     double* best_ms, double* all_ms, void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate_bve_fields(f);
+    check_precision(precision);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const int family =
+        precision == LPM_MIXED ? KID_BVE_VEL_MIXED : KID_BVE_VEL_F64;
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    const EvalFn ev = single_eval(ctx, *f, precision, flags, s);
+    autotune(ctx, family, f->n_verts + f->n_faces, repeats,
+             [&]() { ev(map, vert_crd(*f), face_crd(*f), f->face_zeta); }, s,
+             best_variant, best_ms, all_ms);
+    return LPM_OK;
+  } catch (const std::exception& e) {
+    return fail(ctx, e);
   }
 }
 
 int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                      double Omega, int precision, unsigned flags,
                      void* stream) {
-  bool capturing = false;
-  hipStream_t s = as_stream(stream);
   try {
     LPM_ARG_CHECK(ctx, "null context");
-    validate(f);
-    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
-                  "unknown precision");
+    validate_bve_fields(f);
+    check_precision(precision);
     LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
                   "dt and Omega must be finite");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
-    StageBufs V = carve(ctx->rk_v, f->n_verts);
-    StageBufs F = carve(ctx->rk_f, f->n_faces);
-    if (flags & LPM_RK4_USE_GRAPH) {
-      LPM_ARG_CHECK(s != nullptr,
-                    "LPM_RK4_USE_GRAPH needs a non-default stream");
-      const std::string key = graph_key(*f, dt, Omega, precision, flags, s);
-      auto it = ctx->graphs.find(key);
-      if (it != ctx->graphs.end() &&
-          it->second->epoch != g_workspace_epoch.load()) {
-        ctx->graphs.erase(it);
-        it = ctx->graphs.end();
-      }
-      if (it == ctx->graphs.end()) {
-        // make sure every workspace exists before capture (no allocation
-        // may happen while the stream is being captured)
-        {
-          const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
-          pack_sources(ctx, CrdIn{f->face_x, f->face_x_es, f->face_x_cs},
-                       f->face_zeta, f->face_area, map, precision, s);
-          LPM_HIP_CHECK(hipStreamSynchronize(s));
-        }
-        auto entry = std::make_shared<lpm_ctx::GraphEntry>();
-        const bool timing = ctx->timing;
-        ctx->timing = false;
-        LPM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        capturing = true;
-        try {
-          issue_step(ctx, *f, dt, Omega, precision, flags, V, F, s);
-        } catch (...) {
-          ctx->timing = timing;
-          throw;
-        }
-        ctx->timing = timing;
-        capturing = false;
-        LPM_HIP_CHECK(hipStreamEndCapture(s, &entry->graph));
-        LPM_HIP_CHECK(
-            hipGraphInstantiate(&entry->exec, entry->graph, nullptr, nullptr, 0));
-        entry->epoch = g_workspace_epoch.load();
-        it = ctx->graphs.emplace(key, entry).first;
-      }
-      LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
-    } else {
-      issue_step(ctx, *f, dt, Omega, precision, flags, V, F, s);
-    }
-    if (flags & LPM_FLAG_CHECK_FINITE) {
-      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
-      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
-    }
+    hipStream_t s = as_stream(stream);
+    auto prepare = [&]() -> uint64_t {
+      const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+      pack_sources(ctx, face_crd(*f), f->face_zeta, f->face_area, map,
+                   precision, s);
+      return 0;
+    };
+    run_step(ctx, *f, dt, Omega, precision, flags, s, prepare,
+             single_eval(ctx, *f, precision, flags, s));
+    check_outputs(ctx, *f, flags, s);
     return LPM_OK;
-  } catch (const Error& e) {
-    if (capturing) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-    }
-    set_error(ctx, e.what());
-    return e.code;
   } catch (const std::exception& e) {
-    if (capturing) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-    }
-    set_error(ctx, e.what());
-    return LPM_ERR_INTERNAL;
+    return fail(ctx, e);
+  }
+}
+
+int lpm_bve_ensemble_init_velocity(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                                   int precision, unsigned flags,
+                                   void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(e, "null ensemble");
+    validate_bve_fields(&e->fields);
+    const EnsembleDesc d = validate_ensemble(e);
+    check_precision(precision);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_bve_fields& f = e->fields;
+    const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+    ensemble_eval(ctx, d, precision, flags, s)(map, vert_crd(f), face_crd(f),
+                                               f.face_zeta);
+    check_outputs(ctx, f, flags, s);
+    return LPM_OK;
+  } catch (const std::exception& ex) {
+    return fail(ctx, ex);
+  }
+}
+
+int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                              double dt, double Omega, int precision,
+                              unsigned flags, void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(e, "null ensemble");
+    validate_bve_fields(&e->fields);
+    const EnsembleDesc d = validate_ensemble(e);
+    check_precision(precision);
+    LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
+                  "dt and Omega must be finite");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_bve_fields& f = e->fields;
+    auto prepare = [&]() -> uint64_t {
+      build_rank(ctx, f.face_mask, f.n_faces, s);
+      return ensemble_prepare(ctx, d, precision);
+    };
+    run_step(ctx, f, dt, Omega, precision, flags, s, prepare,
+             ensemble_eval(ctx, d, precision, flags, s));
+    check_outputs(ctx, f, flags, s);
+    return LPM_OK;
+  } catch (const std::exception& ex) {
+    return fail(ctx, ex);
+  }
+}
+
+int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
+                              int precision, unsigned flags, int repeats,
+                              int* best_variant, double* best_ms,
+                              double* all_ms, void* stream) {
+  try {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(e, "null ensemble");
+    validate_bve_fields(&e->fields);
+    const EnsembleDesc d = validate_ensemble(e);
+    check_precision(precision);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_bve_fields& f = e->fields;
+    const int family =
+        precision == LPM_MIXED ? KID_BVE_ENS_MIXED : KID_BVE_ENS_F64;
+    const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+    const EvalFn ev = ensemble_eval(ctx, d, precision, flags, s);
+    autotune(ctx, family, f.n_verts + f.n_faces, repeats,
+             [&]() { ev(map, vert_crd(f), face_crd(f), f.face_zeta); }, s,
+             best_variant, best_ms, all_ms);
+    return LPM_OK;
+  } catch (const std::exception& ex) {
+    return fail(ctx, ex);
   }
 }
 

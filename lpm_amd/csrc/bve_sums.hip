@@ -19,17 +19,17 @@
 #include <algorithm>
 #include <cmath>
 
+#include "bve_device.hpp"
 #include "internal.hpp"
 
 namespace lpm_amd {
 
 namespace {
 
-constexpr double kInv4Pi = 0.079577471545947667884441881686257181;
-constexpr int kPackTile = 256;  // mixed-precision packing tile (sources)
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;  // elements per thread in the mask scan
 constexpr int kScanChunk = kScanBlock * kScanItems;
+static_assert(kScanChunk == kMaskScanChunk, "scan chunk must match the header");
 
 // ------------------------------------------------------------ prologue ---
 __global__ __launch_bounds__(kScanBlock) void mask_count_kernel(
@@ -130,346 +130,30 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(
   out[r] = make_double4(p[0], p[cs], p[2 * cs], -zeta[i] * area[i] * kInv4Pi);
 }
 
-// One block per packing tile: FP64 centre = first source of the tile.
-__global__ __launch_bounds__(kPackTile) void pack_mixed_kernel(
-    const double4* __restrict__ p64, const int32_t* __restrict__ count,
-    float4* __restrict__ out, double4* __restrict__ centers) {
-  const int n = *count;
-  const int j0 = blockIdx.x * kPackTile;
-  if (j0 >= n) return;
-  const double4 c = p64[j0];
-  if (threadIdx.x == 0) centers[blockIdx.x] = c;
-  const int j = j0 + threadIdx.x;
-  if (j < n) {
-    const double4 y = p64[j];
-    out[j] = make_float4(static_cast<float>(y.x - c.x),
-                         static_cast<float>(y.y - c.y),
-                         static_cast<float>(y.z - c.z),
-                         static_cast<float>(2.0 * y.w));
-  }
-}
-
-// ----------------------------------------------------------- arithmetic ---
-// Reciprocal: hardware estimate refined by two Newton-Raphson steps
-// (r <- r + r (1 - d r)), which restores full FP64 accuracy.
-__device__ __forceinline__ double rcp_f64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = __builtin_fma(-d, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-d, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  return r;
-}
-
-// ------------------------------------------------------- kernel argument ---
-struct SegArgs {
-  const double* x;
-  int64_t es, cs, n;
-  const int32_t* index;
-  const int32_t* d_n;
-  const int32_t* self_rank;
-  int self_identity;
-  int nblocks;
-  const uint8_t* skip;
-  double* out;
-  int64_t oes, ocs;
-  double* out_scalar;
-};
-
-__device__ __forceinline__ const SegArgs& pick(const SegArgs& a,
-                                               const SegArgs& b, bool first) {
-  return first ? a : b;
-}
-
-// --------------------------------------------------------- pair bodies ---
-// Reciprocal with NR Newton-Raphson refinements of the hardware estimate.
-template <int NR>
-__device__ __forceinline__ double rcp_nr(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-#pragma unroll
-  for (int it = 0; it < NR; ++it) {
-    const double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-  }
-  return r;
-}
-
-// One source: S += s / (1 - x.y [+ eps^2]) * y.
-template <bool EPS, int NR>
-__device__ __forceinline__ void f64_one(const double* x, const double4& y,
-                                        double eps2, double* S) {
-  const double t = x[0] * y.x + x[1] * y.y + x[2] * y.z;
-  double d = 1.0 - t;
-  if (EPS) d += eps2;
-  const double w = y.w * rcp_nr<NR>(d);
-  S[0] = __builtin_fma(w, y.x, S[0]);
-  S[1] = __builtin_fma(w, y.y, S[1]);
-  S[2] = __builtin_fma(w, y.z, S[2]);
-}
-
-// Two sources sharing one reciprocal: r = 1/(da db), 1/da = db r,
-// 1/db = da r.  Terms are accumulated in source order.
-template <bool EPS, int NR>
-__device__ __forceinline__ void f64_two(const double* x, const double4& ya,
-                                        const double4& yb, double eps2,
-                                        double* S) {
-  const double ta = x[0] * ya.x + x[1] * ya.y + x[2] * ya.z;
-  const double tb = x[0] * yb.x + x[1] * yb.y + x[2] * yb.z;
-  double da = 1.0 - ta, db = 1.0 - tb;
-  if (EPS) {
-    da += eps2;
-    db += eps2;
-  }
-  const double r = rcp_nr<NR>(da * db);
-  const double wa = ya.w * (db * r);
-  const double wb = yb.w * (da * r);
-  S[0] = __builtin_fma(wa, ya.x, S[0]);
-  S[1] = __builtin_fma(wa, ya.y, S[1]);
-  S[2] = __builtin_fma(wa, ya.z, S[2]);
-  S[0] = __builtin_fma(wb, yb.x, S[0]);
-  S[1] = __builtin_fma(wb, yb.y, S[1]);
-  S[2] = __builtin_fma(wb, yb.z, S[2]);
-}
-
-// Mixed: d = y' - x' (tile-relative FP32), r2 = |d|^2 + 2 eps^2,
-// T += (2 s / r2) d.
-template <bool EPS>
-__device__ __forceinline__ void f32_one(const float* x, const float4& y,
-                                        float e2, float* T) {
-  const float d0 = y.x - x[0], d1 = y.y - x[1], d2 = y.z - x[2];
-  float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-  if (EPS) r2 += e2;
-  const float w = y.w * __builtin_amdgcn_rcpf(r2);
-  T[0] = __builtin_fmaf(w, d0, T[0]);
-  T[1] = __builtin_fmaf(w, d1, T[1]);
-  T[2] = __builtin_fmaf(w, d2, T[2]);
-}
-
-template <bool EPS>
-__device__ __forceinline__ void f32_two(const float* x, const float4& ya,
-                                        const float4& yb, float e2, float* T) {
-  const float a0 = ya.x - x[0], a1 = ya.y - x[1], a2 = ya.z - x[2];
-  const float b0 = yb.x - x[0], b1 = yb.y - x[1], b2 = yb.z - x[2];
-  float ra = a0 * a0 + a1 * a1 + a2 * a2;
-  float rb = b0 * b0 + b1 * b1 + b2 * b2;
-  if (EPS) {
-    ra += e2;
-    rb += e2;
-  }
-  const float r = __builtin_amdgcn_rcpf(ra * rb);
-  const float wa = ya.w * (rb * r);
-  const float wb = yb.w * (ra * r);
-  T[0] = __builtin_fmaf(wa, a0, T[0]);
-  T[1] = __builtin_fmaf(wa, a1, T[1]);
-  T[2] = __builtin_fmaf(wa, a2, T[2]);
-  T[0] = __builtin_fmaf(wb, b0, T[0]);
-  T[1] = __builtin_fmaf(wb, b1, T[1]);
-  T[2] = __builtin_fmaf(wb, b2, T[2]);
-}
-
-// Loads the TPL targets of this thread; returns false if the whole block
-// is idle.  Slot c maps to target i = index ? index[c] : c.
-template <int BLOCK, int TPL>
-__device__ __forceinline__ bool load_targets(const SegArgs& sg, int blk,
-                                             double (*x)[3], int* self,
-                                             bool* live, int64_t* tgt) {
-  const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
-  const int64_t slot0 = static_cast<int64_t>(blk) * (BLOCK * TPL);
-  if (slot0 >= n_live) return false;
-#pragma unroll
-  for (int k = 0; k < TPL; ++k) {
-    const int64_t c = slot0 + k * BLOCK + threadIdx.x;
-    const bool in = c < n_live;
-    const int64_t i = in ? (sg.index ? sg.index[c] : c) : 0;
-    tgt[k] = i;
-    live[k] = in && !(sg.skip && sg.skip[i]);
-    const double* p = sg.x + i * sg.es;
-    x[k][0] = p[0];
-    x[k][1] = p[sg.cs];
-    x[k][2] = p[2 * sg.cs];
-    self[k] = !in ? -1
-                  : (sg.self_rank ? sg.self_rank[i]
-                                  : (sg.self_identity ? static_cast<int>(c) : -1));
-  }
-  return true;
-}
-
-// u = x cross S at each live target.
-template <int TPL>
-__device__ __forceinline__ void store_cross(const SegArgs& sg,
-                                            const double (*x)[3],
-                                            const double (*S)[3],
-                                            const bool* live,
-                                            const int64_t* tgt) {
-#pragma unroll
-  for (int k = 0; k < TPL; ++k) {
-    if (!live[k]) continue;
-    double* o = sg.out + tgt[k] * sg.oes;
-    o[0] = x[k][1] * S[k][2] - x[k][2] * S[k][1];
-    o[sg.ocs] = x[k][2] * S[k][0] - x[k][0] * S[k][2];
-    o[2 * sg.ocs] = x[k][0] * S[k][1] - x[k][1] * S[k][0];
-  }
-}
-
-// ---------------------------------------------------------- FP64 kernel ---
-// PAIR: share one reciprocal between two sources in the inner loop.
+// ------------------------------------------------------- velocity sums ---
+// One launch covers two target segments (vertices, then faces): blocks
+// [0, s0.nblocks) take s0, the rest s1.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR>
 __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
     SegArgs s0, SegArgs s1, double eps2) {
-  static_assert(TILE % 8 == 0, "tile must be a multiple of the unroll");
   __shared__ double4 tile[TILE];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  const int nsrc = *count;
-
-  double x[TPL][3], S[TPL][3];
-  int self[TPL];
-  bool live[TPL];
-  int64_t tgt[TPL];
-  if (!load_targets<BLOCK, TPL>(sg, blk, x, self, live, tgt)) return;
-#pragma unroll
-  for (int k = 0; k < TPL; ++k) S[k][0] = S[k][1] = S[k][2] = 0.0;
-
-  for (int j0 = 0; j0 < nsrc; j0 += TILE) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < TILE; t += BLOCK) {
-      const int j = j0 + t;
-      // padding: zero strength at the origin -> 1 - x.y = 1, w = 0
-      tile[t] = j < nsrc ? src[j] : make_double4(0.0, 0.0, 0.0, 0.0);
-    }
-    __syncthreads();
-    bool diag = false;
-#pragma unroll
-    for (int k = 0; k < TPL; ++k)
-      diag |= static_cast<unsigned>(self[k] - j0) < static_cast<unsigned>(TILE);
-    if (__ballot(diag)) {
-      // tile holds some target's own source: per-pair exclusion
-      for (int jj = 0; jj < TILE; ++jj) {
-        const double4 y = tile[jj];
-#pragma unroll
-        for (int k = 0; k < TPL; ++k) {
-          const double t = x[k][0] * y.x + x[k][1] * y.y + x[k][2] * y.z;
-          double d = 1.0 - t;
-          if (EPS) d += eps2;
-          double w = y.w * rcp_nr<2>(d);
-          w = (j0 + jj == self[k]) ? 0.0 : w;
-          S[k][0] = __builtin_fma(w, y.x, S[k][0]);
-          S[k][1] = __builtin_fma(w, y.y, S[k][1]);
-          S[k][2] = __builtin_fma(w, y.z, S[k][2]);
-        }
-      }
-    } else if (PAIR) {
-#pragma unroll 2
-      for (int jj = 0; jj < TILE; jj += 2) {
-        const double4 ya = tile[jj];
-        const double4 yb = tile[jj + 1];
-#pragma unroll
-        for (int k = 0; k < TPL; ++k) f64_two<EPS, NR>(x[k], ya, yb, eps2, S[k]);
-      }
-    } else {
-#pragma unroll 4
-      for (int jj = 0; jj < TILE; ++jj) {
-        const double4 y = tile[jj];
-#pragma unroll
-        for (int k = 0; k < TPL; ++k) f64_one<EPS, NR>(x[k], y, eps2, S[k]);
-      }
-    }
-  }
-  store_cross<TPL>(sg, x, S, live, tgt);
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR>(src, *count, sg, blk, eps2, tile);
 }
 
-// --------------------------------------------------------- mixed kernel ---
-// TILE is a multiple of the packing tile so that one LDS tile spans whole
-// packing tiles; the target offset x - c is formed in FP64 once per packing
-// tile and rounded to FP32.  Per packing tile the FP32 partial is carried
-// into FP64 accumulators.
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
 __global__ __launch_bounds__(BLOCK) void bve_vel_mixed_kernel(
     const float4* __restrict__ src, const double4* __restrict__ centers,
     const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, float eps2x2) {
-  static_assert(TILE % kPackTile == 0, "tile must cover whole packing tiles");
   __shared__ float4 tile[TILE];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  const int nsrc = *count;
-
-  double x[TPL][3], S[TPL][3];
-  int self[TPL];
-  bool live[TPL];
-  int64_t tgt[TPL];
-  if (!load_targets<BLOCK, TPL>(sg, blk, x, self, live, tgt)) return;
-#pragma unroll
-  for (int k = 0; k < TPL; ++k) S[k][0] = S[k][1] = S[k][2] = 0.0;
-
-  for (int j0 = 0; j0 < nsrc; j0 += TILE) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < TILE; t += BLOCK) {
-      const int j = j0 + t;
-      // padding: far away, zero strength -> contributes exactly 0
-      tile[t] = j < nsrc ? src[j] : make_float4(8.0f, 8.0f, 8.0f, 0.0f);
-    }
-    __syncthreads();
-    bool diag = false;
-#pragma unroll
-    for (int k = 0; k < TPL; ++k)
-      diag |= static_cast<unsigned>(self[k] - j0) < static_cast<unsigned>(TILE);
-    const bool any_diag = __ballot(diag) != 0;
-
-    for (int p0 = 0; p0 < TILE && j0 + p0 < nsrc; p0 += kPackTile) {
-      const double4 c = centers[(j0 + p0) / kPackTile];
-      float xr[TPL][3], T[TPL][3];
-#pragma unroll
-      for (int k = 0; k < TPL; ++k) {
-        xr[k][0] = static_cast<float>(x[k][0] - c.x);
-        xr[k][1] = static_cast<float>(x[k][1] - c.y);
-        xr[k][2] = static_cast<float>(x[k][2] - c.z);
-        T[k][0] = T[k][1] = T[k][2] = 0.0f;
-      }
-      if (any_diag) {
-        for (int jj = 0; jj < kPackTile; ++jj) {
-          const float4 y = tile[p0 + jj];
-#pragma unroll
-          for (int k = 0; k < TPL; ++k) {
-            const float d0 = y.x - xr[k][0];
-            const float d1 = y.y - xr[k][1];
-            const float d2 = y.z - xr[k][2];
-            float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (EPS) r2 += eps2x2;
-            float w = y.w * __builtin_amdgcn_rcpf(r2);
-            w = (j0 + p0 + jj == self[k]) ? 0.0f : w;
-            T[k][0] = __builtin_fmaf(w, d0, T[k][0]);
-            T[k][1] = __builtin_fmaf(w, d1, T[k][1]);
-            T[k][2] = __builtin_fmaf(w, d2, T[k][2]);
-          }
-        }
-      } else if (PAIR) {
-#pragma unroll 4
-        for (int jj = 0; jj < kPackTile; jj += 2) {
-          const float4 ya = tile[p0 + jj];
-          const float4 yb = tile[p0 + jj + 1];
-#pragma unroll
-          for (int k = 0; k < TPL; ++k) f32_two<EPS>(xr[k], ya, yb, eps2x2, T[k]);
-        }
-      } else {
-#pragma unroll 8
-        for (int jj = 0; jj < kPackTile; ++jj) {
-          const float4 y = tile[p0 + jj];
-#pragma unroll
-          for (int k = 0; k < TPL; ++k) f32_one<EPS>(xr[k], y, eps2x2, T[k]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < TPL; ++k) {
-        S[k][0] += static_cast<double>(T[k][0]);
-        S[k][1] += static_cast<double>(T[k][1]);
-        S[k][2] += static_cast<double>(T[k][2]);
-      }
-    }
-  }
-  store_cross<TPL>(sg, x, S, live, tgt);
+  mixed_block<BLOCK, TPL, TILE, EPS, PAIR>(src, centers, *count, sg, blk,
+                                           eps2x2, tile);
 }
 
 // ------------------------------------------------------ stream function ---
@@ -487,13 +171,20 @@ __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
   if (static_cast<int64_t>(blk) * BLOCK >= n_live) return;
   const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
   const bool in = c < n_live;
-  const int64_t i = in ? (sg.index ? sg.index[c] : c) : 0;
+  const int64_t i =
+      in ? (sg.index ? sg.index[c] : sg.slot_base + c) : sg.slot_base;
   const bool live = in && !(sg.skip && sg.skip[i]);
   const double* p = sg.x + i * sg.es;
   const double x0 = p[0], x1 = p[sg.cs], x2 = p[2 * sg.cs];
-  const int self = !in ? -1
-                       : (sg.self_rank ? sg.self_rank[i]
-                                       : (sg.self_identity ? static_cast<int>(c) : -1));
+  int self = -1;
+  if (in) {
+    if (sg.self_rank) {
+      const int r = sg.self_rank[i];
+      self = r < 0 ? -1 : r - sg.rank_base;
+    } else if (sg.self_identity) {
+      self = static_cast<int>(c);
+    }
+  }
   double psi = 0.0;
   for (int j0 = 0; j0 < nsrc; j0 += TILE) {
     __syncthreads();
@@ -580,16 +271,22 @@ constexpr int kDefaultMixed = 0;
 int variant_count(int family) {
   if (family == KID_BVE_VEL_F64) return kNumF64;
   if (family == KID_BVE_VEL_MIXED) return kNumMixed;
+  if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
+    return ens_variant_count(family);
   return 0;
 }
 
 const char* variant_name(int family, int v) {
   if (v < 0 || v >= variant_count(family)) return nullptr;
+  if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
+    return ens_variant_name(family, v);
   return family == KID_BVE_VEL_F64 ? kF64Variants[v].name
                                    : kMixedVariants[v].name;
 }
 
 int default_variant(int family) {
+  if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
+    return ens_default_variant(family);
   return family == KID_BVE_VEL_F64 ? kDefaultF64 : kDefaultMixed;
 }
 
@@ -654,6 +351,7 @@ RankMap build_rank(lpm_ctx* ctx, const uint8_t* mask, int64_t n,
   map.rank = rank;
   map.inv = inv;
   map.d_count = counts + nb;
+  map.block_offsets = counts;
   return map;
 }
 

@@ -64,13 +64,19 @@ struct CrdOut {
   int64_t es, cs;
 };
 
+// Elements per block of the mask scan.
+constexpr int kMaskScanChunk = 1024;
+
 // Compaction of a face mask: rank[i] = packed index of face i (-1 if
-// divided), inv[r] = face of packed index r, *d_count = number packed.
+// divided), inv[r] = face of packed index r, *d_count = number packed,
+// block_offsets[b] = number of undivided faces before element
+// b * kMaskScanChunk (b = 0 .. ceil(n / kMaskScanChunk)).
 // With no mask, rank and inv are the identity and not materialised.
 struct RankMap {
   const int32_t* rank = nullptr;
   const int32_t* inv = nullptr;
   const int32_t* d_count = nullptr;
+  const int32_t* block_offsets = nullptr;
   int64_t n = 0;
 };
 
@@ -105,8 +111,9 @@ enum KernelId : int {
   KID_BVE_PSI = 2,
   KID_SWE_F64 = 3,
   KID_SWE_MIXED = 4,
-  KID_BVE_BATCHED = 5,
-  KID_COUNT = 6
+  KID_BVE_ENS_F64 = 5,
+  KID_BVE_ENS_MIXED = 6,
+  KID_COUNT = 7
 };
 
 struct TimingRec {
@@ -142,6 +149,15 @@ struct lpm_ctx {
   lpm_amd::DevBuf swe_packed;   // SWE sources (64 B each)
   lpm_amd::DevBuf swe_bounds;   // SWE tile bounding spheres
 
+  // ensemble plan and its device tables
+  struct EnsemblePlan;
+  std::shared_ptr<EnsemblePlan> ens_plan;
+  uint64_t ens_plan_serial = 0;
+  lpm_amd::DevBuf ens_members;  // per member: offsets, packed ranges
+  lpm_amd::DevBuf ens_items;    // int4 work items
+  lpm_amd::DevBuf ens_bounds;   // int32 per member boundary
+  lpm_amd::DevBuf ens_queue;    // int32 work-queue head (+ padded total)
+
   // RK4 stage buffers
   lpm_amd::DevBuf rk_v, rk_f;   // BVE vertex / face stage storage
   lpm_amd::DevBuf swe_v, swe_f; // SWE vertex / face stage storage
@@ -157,7 +173,7 @@ struct lpm_ctx {
 
   // autotuned kernel variants, keyed by (arch, kernel, size bucket)
   std::map<std::string, int> tuned;
-  int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1};
+  int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1, -1};
 };
 
 namespace lpm_amd {
@@ -200,6 +216,29 @@ void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s);
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
                   int64_t cs, int ncomp, hipStream_t s);
+
+// ---- RK4 (bve_rk4.hip) ----
+void validate_bve_fields(const lpm_bve_fields* f);
+
+// ---- ensembles (bve_ensemble.hip) ----
+// Validated host view of an lpm_bve_ensemble.
+struct EnsembleDesc {
+  const lpm_bve_ensemble* e = nullptr;
+  int64_t max_member_faces = 0;
+  int64_t total_padded_upper = 0;  // sum of member faces rounded up to 256
+};
+EnsembleDesc validate_ensemble(const lpm_bve_ensemble* e);
+int ens_variant_count(int family);
+const char* ens_variant_name(int family, int v);
+int ens_default_variant(int family);
+// Allocates every workspace an evaluation uses, so that a stream capture
+// that follows allocates nothing; returns the serial of the work plan.
+uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d, int precision);
+// Velocity of every member from positions (vx, fx) and face vorticity
+// fzeta, written to the ensemble's vert_u / face_u.
+void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d, CrdIn vx,
+                       CrdIn fx, const double* fzeta, const RankMap& map,
+                       int precision, unsigned flags, hipStream_t s);
 
 // ---- sphere SWE sums (swe_sums.hip) ----
 struct SwePacked {

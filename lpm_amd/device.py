@@ -117,6 +117,36 @@ class Context:
             precision_id(precision), flags,
             stream if stream is not None else current_stream()))
 
+    # ---------------------------------------------------------- ensembles
+    def bve_ensemble_init_velocity(self, ens, precision="fp64", flags=0,
+                                   stream=None):
+        self.check(self.L.lpm_bve_ensemble_init_velocity(
+            self.h, ctypes.byref(ens), precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
+    def bve_ensemble_rk4_step(self, ens, dt, Omega, precision="fp64", flags=0,
+                              stream=None):
+        self.check(self.L.lpm_bve_ensemble_rk4_step(
+            self.h, ctypes.byref(ens), float(dt), float(Omega),
+            precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
+    def bve_ensemble_autotune(self, ens, precision="fp64", flags=0, repeats=2,
+                              stream=None):
+        """Returns (best variant, its ms, list of ms of every variant)."""
+        kid = (_abi.LPM_KERNEL_BVE_ENSEMBLE_F64
+               if precision_id(precision) == _abi.LPM_FP64
+               else _abi.LPM_KERNEL_BVE_ENSEMBLE_MIXED)
+        nv = self.L.lpm_variant_count(kid)
+        best = ctypes.c_int()
+        ms = ctypes.c_double()
+        all_ms = (ctypes.c_double * nv)()
+        self.check(self.L.lpm_bve_ensemble_autotune(
+            self.h, ctypes.byref(ens), precision_id(precision), flags,
+            repeats, ctypes.byref(best), ctypes.byref(ms), all_ms,
+            stream if stream is not None else current_stream()))
+        return best.value, ms.value, list(all_ms)
+
     # ----------------------------------------------------------- SWE sums
     def swe_sums(self, tgt_x, tgt_surf, src_x, src_zeta, src_sigma, src_area,
                  src_surf, src_mask=None, collocated=False, eps=0.0,

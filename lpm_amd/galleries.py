@@ -144,3 +144,42 @@ class SplitMix64:
             z = z ^ (z >> np.uint64(31))
         self.state = (self.state + n * 0x9E3779B97F4A7C15) & _MASK64
         return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def shoemake_rotation(u1, u2, u3):
+    """Rotation matrix of the unit quaternion drawn from three uniforms by
+    Shoemake's method (uniform over SO(3)):
+    q = (sqrt(1-u1) sin 2pi u2, sqrt(1-u1) cos 2pi u2,
+         sqrt(u1) sin 2pi u3, sqrt(u1) cos 2pi u3) = (x, y, z, w)."""
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    qx, qy = a * np.sin(TWO_PI * u2), a * np.cos(TWO_PI * u2)
+    qz, qw = b * np.sin(TWO_PI * u3), b * np.cos(TWO_PI * u3)
+    return np.array([
+        [1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+        [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+        [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)],
+    ])
+
+
+ENSEMBLE_SEED = 0x4C504D34
+
+
+def ensemble_member(m: int, seed_base: int = ENSEMBLE_SEED, levels=(3, 6)):
+    """Parameters of ensemble member m: icosahedral level
+    L = lo + (r mod (hi - lo + 1)), Rossby-Haurwitz 5-4 with omega and K
+    uniform in [0.05, 0.15), and a uniformly random rotation R of the
+    pattern; all drawn from splitmix64(seed_base + m) in that order."""
+    rng = SplitMix64(seed_base + m)
+    lo, hi = levels
+    level = lo + rng.next_u64() % (hi - lo + 1)
+    omega = 0.05 + 0.1 * rng.uniform()
+    K = 0.05 + 0.1 * rng.uniform()
+    R = shoemake_rotation(rng.uniform(), rng.uniform(), rng.uniform())
+    return {"level": int(level), "omega": omega, "K": K, "rotation": R}
+
+
+def rotated_rh54_vorticity(x, omega, K, R):
+    """Rossby-Haurwitz 5-4 vorticity (u0 = omega, amp = -K) of the pattern
+    rotated by R: zeta(x) = zeta_RH(R^T x)."""
+    x = np.asarray(x, dtype=np.float64)
+    return rh54_vorticity(R.T @ x, omega, -K)
