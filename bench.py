@@ -55,6 +55,15 @@ def errors(a, b):
     return float(np.sqrt((e ** 2).sum() / (r ** 2).sum())), float(e.max() / r.max())
 
 
+def errors1(a, b):
+    """Relative L2 and Linf of scalar fields."""
+    import numpy as np
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return (float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())),
+            float(np.abs(a - b).max() / np.abs(b).max()))
+
+
 def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra,
                   autotune=True):
     import numpy as np
@@ -104,6 +113,285 @@ def run_precision(ctx, mesh, precision, steps, warmup, dt, flags_extra,
            "uT_v": s.velocity_verts.cpu().numpy(),
            "xT_v": s.vert_x.cpu().numpy()}
     del s, solver
+    return out
+
+
+def ensemble_run(ctx, steps, warmup, dt=0.01, autotune=True):
+    """64-member ensemble (icosahedral levels 3-6, RH 5-4 with random
+    omega, K and pattern rotation), one persistent launch per evaluation,
+    graph-captured RK4; mixed precision timed, FP64 for the error."""
+    import numpy as np
+    import torch
+    from lpm_amd import _abi
+    from lpm_amd import galleries as G
+    from lpm_amd.bve import BVEEnsemble, BVEEnsembleRK4
+    from lpm_amd.mesh import SphereMesh
+
+    params = [G.ensemble_member(m) for m in range(64)]
+    cache = {}
+    meshes = []
+    for p in params:
+        if p["level"] not in cache:
+            cache[p["level"]] = SphereMesh.build("icos", p["level"])
+        meshes.append(cache[p["level"]])
+    fns = [(lambda x, p=p: G.rotated_rh54_vorticity(x, p["omega"], p["K"],
+                                                    p["rotation"]))
+           for p in params]
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    out = {"members": 64,
+           "levels": {str(L): sum(1 for p in params if p["level"] == L)
+                      for L in sorted(cache)}}
+    pairs = None
+    res = {}
+    stream = torch.cuda.Stream()
+    for prec in ("mixed", "fp64"):
+        ens = BVEEnsemble(meshes, ctx=ctx)
+        ens.init_vorticity(fns)
+        pairs = ens.useful_pairs()
+        kid = (_abi.LPM_KERNEL_BVE_ENSEMBLE_MIXED if prec == "mixed"
+               else _abi.LPM_KERNEL_BVE_ENSEMBLE_F64)
+        names = ctx.variants(kid)
+        tune = None
+        if autotune:
+            best, best_ms, all_ms = ens.autotune(prec, skip, repeats=2)
+            d = ctx.L.lpm_variant_default(kid)
+            tune = {"variant": names[best], "ms": round(best_ms, 3),
+                    "fixed_tile_baseline": {"variant": names[d],
+                                            "ms": round(all_ms[d], 3)},
+                    "all_ms": {n: round(t, 3) for n, t in zip(names, all_ms)}}
+            log(f"ensemble autotune {prec}: {names[best]} {best_ms:.2f} ms "
+                f"(fixed {names[d]} {all_ms[d]:.2f} ms)")
+        ens.init_velocity(prec, skip)
+        torch.cuda.synchronize()
+        u0 = ens.velocity_verts.cpu().numpy()
+        st = BVEEnsembleRK4(dt, ens, 2 * np.pi, prec, reference_compat=True,
+                            renormalize=True, leaf_only=True, use_graph=True,
+                            stream=stream.cuda_stream)
+        for _ in range(warmup):
+            st.advance_timestep(ens)
+        stream.synchronize()
+        ctx.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.advance_timestep(ens)
+        stream.synchronize()
+        el = time.perf_counter() - t0
+        ctx.timing_enable(False)
+        res[prec] = {"elapsed": el, "u0": u0, "tune": tune,
+                     "uT": ens.velocity_verts.cpu().numpy(),
+                     "xT": ens.vert_x.cpu().numpy()}
+        log(f"ensemble {prec}: {el / steps * 1e3:.1f} ms/step, "
+            f"{4 * pairs * steps / el / 1e9:.1f} Gpairs/s")
+        if prec == "mixed":
+            # analytic RH velocity of each member's rotated pattern at t = 0
+            ue = []
+            for m, p in enumerate(params):
+                xm = meshes[m].vert_x
+                ue.append(p["rotation"] @ G.rh54_velocity(
+                    p["rotation"].T @ xm, p["omega"], -p["K"]))
+            out["mixed_vs_analytic_rh54_t0_vertices_rel_l2_linf"] = errors(
+                u0, np.concatenate(ue, axis=1))
+        del ens, st
+    m, f = res["mixed"], res["fp64"]
+    out.update({
+        "pairs_per_evaluation": pairs,
+        "steps": steps, "warmup": warmup, "dt": dt,
+        "value": round(4 * pairs * steps / m["elapsed"] / 1e9, 3),
+        "unit": "Gpair-interactions/s",
+        "ms_per_step": round(m["elapsed"] / steps * 1e3, 3),
+        "rk4_steps_per_s": round(steps / m["elapsed"], 4),
+        "dtype": "f32 pair arithmetic, f64 carries/state",
+        "graph": True,
+        "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
+        "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
+        "mixed_vs_fp64_t0_vertices_rel_l2_linf": errors(m["u0"], f["u0"]),
+        "mixed_vs_fp64_after_steps_vertices_rel_l2_linf": errors(m["uT"], f["uT"]),
+        "mixed_vs_fp64_after_steps_positions_rel_l2_linf": errors(m["xT"], f["xT"]),
+        "autotune_mixed": m["tune"], "autotune_fp64": f["tune"],
+    })
+    return out
+
+
+def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8):
+    """Gaussian vortex on the cubed sphere, level 8 (393,216 faces): mixed
+    against FP64 velocity and positions after 1 and 50 RK4 steps."""
+    import numpy as np
+    import torch
+    from lpm_amd import galleries as G
+    from lpm_amd.bve import BVERK4, BVESphere
+    from lpm_amd.mesh import SphereMesh
+
+    mesh = SphereMesh.build("cube", level)
+    ctr = np.array([np.sqrt(0.5), 0.0, np.sqrt(0.5)])
+    rng = G.SplitMix64(0x4C504D33)
+    noise_f = 1e-3 * (2 * rng.uniform_array(mesh.n_faces) - 1)
+    noise_v = 1e-3 * (2 * rng.uniform_array(mesh.n_verts) - 1)
+
+    def init(s):
+        zf = G.gaussian_vortex_vorticity(mesh.face_x, ctr, 4 * np.pi, 4.0)
+        zv = G.gaussian_vortex_vorticity(mesh.vert_x, ctr, 4 * np.pi, 4.0)
+        s.rel_vort_faces.copy_(torch.from_numpy(zf + noise_f))
+        s.rel_vort_verts.copy_(torch.from_numpy(zv + noise_v))
+
+    snaps = {}
+    times = {}
+    for prec in ("fp64", "mixed"):
+        s = BVESphere("cube", level, ctx=ctx, mesh=mesh)
+        init(s)
+        s.init_velocity(prec, flags=2)
+        solver = BVERK4(dt, s, precision=prec, reference_compat=True,
+                        renormalize=True, leaf_only=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        for n in sorted(steps_list):
+            while done < n:
+                solver.advance_timestep(s)
+                done += 1
+            torch.cuda.synchronize()
+            snaps[(prec, n)] = (s.velocity_verts.cpu().numpy(),
+                                s.vert_x.cpu().numpy())
+        times[prec] = (time.perf_counter() - t0) / done
+        log(f"precision study {prec}: {times[prec] * 1e3:.1f} ms/step")
+        del s, solver
+    out = {"workload": (f"cubed sphere level {level} ({mesh.n_verts} vertices, "
+                        f"{mesh.n_leaves} faces), Gaussian vortex 4 pi exp(-16 "
+                        f"(1 - x.x0)) at 45N 0E + U(-1e-3, 1e-3) noise "
+                        f"(splitmix64 0x4C504D33), Omega = 2 pi, dt = {dt}, "
+                        f"stage positions renormalised"),
+           "ms_per_step": {k: round(v * 1e3, 3) for k, v in times.items()}}
+    for n in sorted(steps_list):
+        uf, xf = snaps[("fp64", n)]
+        um, xm = snaps[("mixed", n)]
+        out[f"after_{n}_steps"] = {
+            "velocity_rel_l2_linf": errors(um, uf),
+            "positions_rel_l2_linf": errors(xm, xf)}
+    return out
+
+
+def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
+                     oracle_stride=64):
+    """Shallow water: 16 steady zonal (TC2) states on the icosahedral mesh of
+    level 6 with seeded 1e-4 height noise, one persistent launch per
+    evaluation, graph-captured RK4; mixed timed, FP64 for the error."""
+    import numpy as np
+    import torch
+    from lpm_amd import _abi
+    from lpm_amd import galleries as G
+    from lpm_amd.mesh import SphereMesh
+    from lpm_amd.swe import SWEEnsemble, SWEEnsembleRK4
+
+    mesh = SphereMesh.build("icos", level)
+    h = mesh.mean_spacing()
+    eps, pse_eps = h, 2 * h
+    noise = []
+    for b in range(members):
+        rng = G.SplitMix64(0x4C504D35 + b)
+        noise.append((1e-4 * (2 * rng.uniform_array(mesh.n_verts) - 1),
+                      1e-4 * (2 * rng.uniform_array(mesh.n_faces) - 1)))
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    st_v, st_f = G.tc2_state(mesh.vert_x), G.tc2_state(mesh.face_x)
+
+    def build():
+        e = SWEEnsemble([mesh] * members, ctx=ctx)
+        sv = np.concatenate([st_v["surface"] + nv for nv, _ in noise])
+        sf = np.concatenate([st_f["surface"] + nf for _, nf in noise])
+        area = np.tile(mesh.face_area, members)
+        for t, v in ((e.surf_passive, sv), (e.depth_passive, sv),
+                     (e.surf_active, sf), (e.depth_active, sf),
+                     (e.mass_active, sf * area),
+                     (e.rel_vort_passive, np.tile(st_v["zeta"], members)),
+                     (e.rel_vort_active, np.tile(st_f["zeta"], members))):
+            t.copy_(torch.from_numpy(v))
+        e.set_kernel_parameters(eps, pse_eps)
+        return e
+
+    stream = torch.cuda.Stream()
+    res = {}
+    pairs = None
+    for prec in ("mixed", "fp64"):
+        e = build()
+        pairs = e.useful_pairs()
+        e.init_direct_sums(prec, skip)
+        torch.cuda.synchronize()
+        t0 = {k: getattr(e, k).cpu().numpy().copy() for k in (
+            "velocity_passive", "double_dot_passive", "surf_lap_passive",
+            "surf_passive", "rel_vort_active", "div_active")}
+        r = SWEEnsembleRK4(dt, e, prec, reference_compat=True, leaf_only=True,
+                           use_graph=True, stream=stream.cuda_stream)
+        for _ in range(warmup):
+            e.advance_timestep(r)
+        stream.synchronize()
+        tb = time.perf_counter()
+        for _ in range(steps):
+            e.advance_timestep(r)
+        stream.synchronize()
+        el = time.perf_counter() - tb
+        tT = {k: getattr(e, k).cpu().numpy().copy() for k in (
+            "velocity_passive", "surf_passive", "vert_x", "div_passive")}
+        res[prec] = {"elapsed": el, "t0": t0, "tT": tT}
+        log(f"swe ensemble {prec}: {el / steps * 1e3:.1f} ms/step, "
+            f"{4 * pairs * steps / el / 1e9:.1f} Gpairs/s")
+        del e, r
+    m, f = res["mixed"], res["fp64"]
+    uex = np.tile(st_v["u"], (1, members))
+    out = {
+        "workload": (f"{members} x shallow water on the icosahedral mesh level "
+                     f"{level} ({mesh.n_verts} vertices, {mesh.n_leaves} faces "
+                     f"each), steady zonal flow u0 = 0.2, g = 1, h0 = 1 (polar "
+                     f"depth), Omega = 2 pi, height noise U(-1e-4, 1e-4) "
+                     f"(splitmix64 0x4C504D35 + b), eps = {eps:.5f}, PSE "
+                     f"width {pse_eps:.5f}, dt = {dt}"),
+        "pairs_per_evaluation": pairs,
+        "steps": steps, "warmup": warmup,
+        "value": round(4 * pairs * steps / m["elapsed"] / 1e9, 3),
+        "unit": "Gpair-interactions/s",
+        "ms_per_step": round(m["elapsed"] / steps * 1e3, 3),
+        "rk4_steps_per_s": round(steps / m["elapsed"], 4),
+        "dtype": "f32 pair arithmetic (PSE terms f64), f64 state",
+        "graph": True,
+        "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
+        "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
+        "mixed_vs_fp64_t0": {
+            "velocity_rel_l2_linf": errors(m["t0"]["velocity_passive"],
+                                           f["t0"]["velocity_passive"]),
+            "ddot_rel_l2_linf": errors1(m["t0"]["double_dot_passive"],
+                                        f["t0"]["double_dot_passive"]),
+            "laps_rel_l2_linf": errors1(m["t0"]["surf_lap_passive"],
+                                        f["t0"]["surf_lap_passive"])},
+        "mixed_vs_fp64_after_steps": {
+            "velocity_rel_l2_linf": errors(m["tT"]["velocity_passive"],
+                                           f["tT"]["velocity_passive"]),
+            "surface_rel_l2_linf": errors1(m["tT"]["surf_passive"],
+                                           f["tT"]["surf_passive"])},
+        "vs_steady_state_t0": {
+            "velocity_rel_l2_linf": errors(m["t0"]["velocity_passive"], uex),
+            "ddot_rel_l2_linf": errors1(m["t0"]["double_dot_passive"],
+                                        np.tile(st_v["ddot"], members)),
+            "laps_rel_l2_linf": errors1(m["t0"]["surf_lap_passive"],
+                                        np.tile(st_v["lap_surface"], members))},
+        "vs_steady_state_after_steps": {
+            "surface_rel_l2_linf": errors1(
+                m["tT"]["surf_passive"], G.tc2_state(m["tT"]["vert_x"])["surface"]),
+            "max_abs_divergence": float(np.abs(m["tT"]["div_passive"]).max())},
+    }
+    # FP64 GPU against the CPU oracle on a sample of member 0's vertices
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sl = slice(0, mesh.n_verts, oracle_stride)
+    sv0 = st_v["surface"] + noise[0][0]
+    sf0 = st_f["surface"] + noise[0][1]
+    ru, rdd, rlap = oracle.swe_sums(
+        np.ascontiguousarray(mesh.vert_x[:, sl].T), sv0[sl],
+        np.ascontiguousarray(mesh.face_x.T), st_f["zeta"], st_f["sigma"],
+        mesh.face_area, sf0, mesh.face_mask, eps=eps, pse_eps=pse_eps,
+        nthreads=min(16, os.cpu_count() or 1))
+    f0 = f["t0"]
+    out["fp64_vs_cpu_oracle_sample"] = {
+        "targets": f"every {oracle_stride}th vertex of member 0",
+        "velocity_rel_l2_linf": errors(f0["velocity_passive"][:, sl], ru.T),
+        "ddot_rel_l2_linf": errors1(f0["double_dot_passive"][sl], rdd),
+        "laps_rel_l2_linf": errors1(f0["surf_lap_passive"][sl], rlap)}
     return out
 
 
@@ -204,6 +492,13 @@ def main(argv=None):
                          "of the FP64 run)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-member ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-precision-study", action="store_true",
+                    help="skip the cubed-sphere mixed-vs-FP64 study")
+    ap.add_argument("--no-swe", action="store_true",
+                    help="skip the 16-state shallow-water ensemble")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -327,6 +622,19 @@ def main(argv=None):
         }
         log(f"mixed: {mstep*1e3:.1f} ms/step, {mval:.1f} Gpairs/s "
             f"({mval/value:.2f}x FP64)")
+
+    if not a.no_ensemble:
+        log("ensemble run (64 members)")
+        result["ensemble"] = ensemble_run(ctx, a.ensemble_steps, 1,
+                                          autotune=not a.no_autotune)
+        result["ensemble"]["speedup_vs_fp64_single_problem"] = round(
+            result["ensemble"]["value"] / value, 4)
+    if not a.no_precision_study:
+        log("precision study (cubed sphere level 8)")
+        result["precision_study"] = precision_study(ctx)
+    if not a.no_swe:
+        log("shallow-water ensemble (16 x level 6)")
+        result["swe_ensemble"] = swe_ensemble_run(ctx)
 
     if not a.no_cpu_baseline:
         log("CPU baseline (oracle) on a bounded sample")

@@ -294,10 +294,32 @@ int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
  * laps and surf must be current (lpm_swe_init_sums or the previous step).
  * LPM_RK4_REFERENCE_COMPAT: the stage-4 position increment stays zero, as
  * in the reference, which never assigns it.  LPM_FLAG_SKIP_MASKED_TARGETS:
- * divided faces are neither evaluated nor advanced. */
+ * divided faces are neither evaluated nor advanced.  LPM_RK4_USE_GRAPH:
+ * the step is captured into a hipGraph on first use and replayed (needs a
+ * non-default stream). */
 int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
                      double Omega, double g, double eps, double pse_eps,
                      int precision, unsigned flags, void* stream);
+
+/* Ensembles of sphere SWE problems stored back to back, as
+ * lpm_bve_ensemble (same offset rules): one persistent launch evaluates the
+ * sums of every member.  No reference counterpart; each member's results
+ * equal lpm_swe_init_sums / lpm_swe_rk4_step on that member alone, bit for
+ * bit. */
+typedef struct lpm_swe_ensemble {
+  int32_t n_members;
+  const int64_t* vert_offsets; /* host, n_members + 1 */
+  const int64_t* face_offsets; /* host, n_members + 1 */
+  lpm_swe_fields fields;       /* concatenated member arrays */
+} lpm_swe_ensemble;
+
+int lpm_swe_ensemble_init_sums(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                               double eps, double pse_eps, int precision,
+                               unsigned flags, void* stream);
+int lpm_swe_ensemble_rk4_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                              double dt, double Omega, double g, double eps,
+                              double pse_eps, int precision, unsigned flags,
+                              void* stream);
 
 /* ---------------------------------------------------- kernel variants ---- */
 /* The direct-sum kernels are compiled in several tile/block/unroll shapes

@@ -95,6 +95,14 @@ class SweFields(ctypes.Structure):
     ]
 
 
+class SweEnsemble(ctypes.Structure):
+    """struct lpm_swe_ensemble (offsets are host int64 arrays)."""
+    _fields_ = [
+        ("n_members", ctypes.c_int32), ("vert_offsets", c_i64p),
+        ("face_offsets", c_i64p), ("fields", SweFields),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "lpm_abi_version": (c_int, []),
@@ -134,6 +142,11 @@ _SIGS = {
                                   c_int, c_uint, c_vp]),
     "lpm_swe_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweFields), c_dbl, c_dbl,
                                  c_dbl, c_dbl, c_dbl, c_int, c_uint, c_vp]),
+    "lpm_swe_ensemble_init_sums": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
+                                           c_dbl, c_dbl, c_int, c_uint, c_vp]),
+    "lpm_swe_ensemble_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
+                                          c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+                                          c_int, c_uint, c_vp]),
     "lpm_variant_count": (c_int, [c_int]),
     "lpm_variant_default": (c_int, [c_int]),
     "lpm_variant_name": (ctypes.c_char_p, [c_int, c_int]),

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 # host-only sources: strict IEEE operation order for the mesh generator
 HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
-HIP_SOURCES = ["bve_sums.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
+HIP_SOURCES = ["bve_sums.hip", "ensemble.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
                "swe_rk4.hip"]
 EXAMPLES = ["bve_rotation", "sphere_swe_tc2"]
 

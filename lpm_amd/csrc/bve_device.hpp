@@ -362,5 +362,239 @@ __global__ __launch_bounds__(kPackTile) void pack_mixed_kernel(
   }
 }
 
+
+// ------------------------------------------------ matrix-core mixed path ---
+// Far-field pairs on the matrix cores, per 256-source tile with FP64 centre
+// c (centroid) and radius R, in coordinates relative to c rounded to FP32:
+//   GEMM1 (v_mfma_f32_16x16x4_f32, 16 sources x 16 targets x K = 4):
+//     r2 = |x'|^2 - 2 x'.y' + |y'|^2  with A_j = (y', 1),
+//     B_i = (-2 x', |x'|^2), C_ji = |y'_j|^2;
+//   VALU: w = 2 s / r2;
+//   GEMM2 (v_mfma_f32_4x4x1_16b_f32): sum_j w (y', 1) per target, i.e. the
+//     tile's sum of w y' and of w, carried into FP64 as S += S' + c W.
+// A wave (64 targets) uses the matrix path for a tile only when none of its
+// targets lies within kMfmaNear R of c: then r2 >= (kMfmaNear - 1)^2 R^2
+// while its FP32 terms are at most (kMfmaNear + 1)^2 R^2, which bounds the
+// relative rounding of r2 by about 12 ulp for kMfmaNear = 3.  Tiles near
+// some target of the wave (including every tile holding a target's own
+// source) take the difference form d = y' - x' on the vector ALUs, as the
+// mixed block does.
+//
+// Packed matrix-core layout (arrays of ntile * 256 entries, padded tiles):
+//   mf_y[j] = (y'_0, y'_1, y'_2, 1)       GEMM1 A and near-path positions
+//   mf_q[j] = |y'|^2                       GEMM1 C
+//   mf_s[j] = 2 s                          weights (0 in padding)
+//   mf_t[t 1024 + ((ch 4 + g) 4 + c) 4 + r] = component c of source
+//            256 t + 16 ch + 4 g + r (c = 3: 1), the GEMM2 A operand
+//   mf_c[t] = (c, R^2)
+// Padding sources sit at y' = (8, 8, 8) with zero weight.
+constexpr double kMfmaNear = 3.0;
+constexpr float kMfmaPad = 8.0f;
+
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+struct MfmaSrc {
+  const float4* y;
+  const float* q;
+  const float* s;
+  const float* t;
+  const double4* c;
+};
+
+// One block per 256-source tile of the FP64 packed array (count entries):
+// centroid, radius and the matrix-core layout.  A source whose FP64 record
+// is all zero is padding (real sources lie on the unit sphere).
+__global__ __launch_bounds__(kPackTile) void pack_mfma_kernel(
+    const double4* __restrict__ p64, const int32_t* __restrict__ count,
+    float4* __restrict__ mf_y, float* __restrict__ mf_q,
+    float* __restrict__ mf_s, float* __restrict__ mf_t,
+    double4* __restrict__ mf_c) {
+  __shared__ double red[4][kPackTile / 64];
+  __shared__ double4 cen;
+  const int n = *count;
+  const int j0 = blockIdx.x * kPackTile;
+  if (j0 >= n) return;
+  const int j = j0 + threadIdx.x;
+  double4 y = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (j < n) y = p64[j];
+  const bool real = (j < n) && !(y.x == 0.0 && y.y == 0.0 && y.z == 0.0 &&
+                                 y.w == 0.0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // centroid of the real sources
+  double v[4] = {real ? y.x : 0.0, real ? y.y : 0.0, real ? y.z : 0.0,
+                 real ? 1.0 : 0.0};
+  for (int k = 0; k < 4; ++k) {
+    double a = v[k];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) red[k][wv] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+      for (int w = 0; w < kPackTile / 64; ++w) sum[k] += red[k][w];
+    const double m = sum[3] > 0 ? sum[3] : 1.0;
+    cen = make_double4(sum[0] / m, sum[1] / m, sum[2] / m, 0.0);
+  }
+  __syncthreads();
+  const double4 c = cen;
+  const double e0 = y.x - c.x, e1 = y.y - c.y, e2 = y.z - c.z;
+  double r2 = real ? e0 * e0 + e1 * e1 + e2 * e2 : 0.0;
+  for (int off = 32; off > 0; off >>= 1) r2 = fmax(r2, __shfl_down(r2, off, 64));
+  __syncthreads();
+  if (lane == 0) red[0][wv] = r2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kPackTile / 64; ++w) m = fmax(m, red[0][w]);
+    mf_c[blockIdx.x] = make_double4(c.x, c.y, c.z, m);
+  }
+  float y0 = kMfmaPad, y1 = kMfmaPad, y2 = kMfmaPad, ws = 0.0f;
+  if (real) {
+    y0 = static_cast<float>(e0);
+    y1 = static_cast<float>(e1);
+    y2 = static_cast<float>(e2);
+    ws = static_cast<float>(2.0 * y.w);
+  }
+  mf_y[j] = make_float4(y0, y1, y2, 1.0f);
+  mf_q[j] = y0 * y0 + y1 * y1 + y2 * y2;
+  mf_s[j] = ws;
+  // transposed GEMM2 operand: local index jj = 16 ch + 4 g + r
+  const int jj = threadIdx.x, ch = jj >> 4, g = (jj >> 2) & 3, r = jj & 3;
+  float* tt = mf_t + static_cast<int64_t>(blockIdx.x) * 1024 + (ch * 4 + g) * 16 + r;
+  tt[0] = y0;
+  tt[4] = y1;
+  tt[8] = y2;
+  tt[12] = 1.0f;
+}
+
+// Block `blk` (256 targets, one per lane) of segment sg against nsrc
+// sources in the matrix-core layout.  Every thread of the block calls it.
+template <bool EPS>
+__device__ __forceinline__ void mfma_block(const MfmaSrc& src, int nsrc,
+                                           const SegArgs& sg, int blk,
+                                           float eps2x2, float4* ly, float* lq,
+                                           float* ls, float4* lt) {
+  double x[1][3];
+  int self[1];
+  bool live[1];
+  int64_t tgt[1];
+  if (!load_targets<256, 1>(sg, blk, x, self, live, tgt)) return;
+  const int lane = threadIdx.x & 63;
+  const int k = lane >> 4;   // MFMA K row of this lane
+  const int col = lane & 15; // MFMA target column
+  // coordinates of the 4 x 16 targets of this wave's matrix columns
+  double tx[4][3];
+#pragma unroll
+  for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tx[tb][a] = __shfl(x[0][a], 16 * tb + col, 64);
+  double Sf[4][3];  // far-field sums per (target column, source group)
+  double Sn[3] = {0.0, 0.0, 0.0};  // near-field sums of this lane's target
+#pragma unroll
+  for (int tb = 0; tb < 4; ++tb) Sf[tb][0] = Sf[tb][1] = Sf[tb][2] = 0.0;
+
+  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+    const int t = j0 / kPackTile;
+    __syncthreads();
+    {
+      const int64_t base = static_cast<int64_t>(t) * kPackTile + threadIdx.x;
+      ly[threadIdx.x] = src.y[base];
+      lq[threadIdx.x] = src.q[base];
+      ls[threadIdx.x] = src.s[base];
+      lt[threadIdx.x] = reinterpret_cast<const float4*>(src.t)[base];
+    }
+    __syncthreads();
+    const double4 c = src.c[t];
+    const double e0 = x[0][0] - c.x, e1 = x[0][1] - c.y, e2 = x[0][2] - c.z;
+    const bool near =
+        e0 * e0 + e1 * e1 + e2 * e2 <= kMfmaNear * kMfmaNear * c.w;
+    if (__ballot(near)) {
+      // difference form on the vector ALUs, one target per lane
+      const float xr0 = static_cast<float>(e0), xr1 = static_cast<float>(e1),
+                  xr2 = static_cast<float>(e2);
+      const bool diag = static_cast<unsigned>(self[0] - j0) <
+                        static_cast<unsigned>(kPackTile);
+      float T0 = 0.0f, T1 = 0.0f, T2 = 0.0f;
+      for (int jj = 0; jj < kPackTile; ++jj) {
+        const float4 y = ly[jj];
+        const float d0 = y.x - xr0, d1 = y.y - xr1, d2 = y.z - xr2;
+        float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (EPS) r2 += eps2x2;
+        float w = ls[jj] * __builtin_amdgcn_rcpf(r2);
+        if (diag && j0 + jj == self[0]) w = 0.0f;
+        T0 = __builtin_fmaf(w, d0, T0);
+        T1 = __builtin_fmaf(w, d1, T1);
+        T2 = __builtin_fmaf(w, d2, T2);
+      }
+      Sn[0] += static_cast<double>(T0);
+      Sn[1] += static_cast<double>(T1);
+      Sn[2] += static_cast<double>(T2);
+      continue;
+    }
+    // matrix-core far field
+    float B[4];
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) {
+      const float a0 = static_cast<float>(tx[tb][0] - c.x);
+      const float a1 = static_cast<float>(tx[tb][1] - c.y);
+      const float a2 = static_cast<float>(tx[tb][2] - c.z);
+      const float n2 = a0 * a0 + a1 * a1 + a2 * a2;
+      B[tb] = k == 0 ? -2.0f * a0 : k == 1 ? -2.0f * a1 : k == 2 ? -2.0f * a2 : n2;
+    }
+    mf4 D2[4];
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) D2[tb] = mf4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float* lyf = reinterpret_cast<const float*>(ly);
+#pragma unroll 2
+    for (int ch = 0; ch < kPackTile / 16; ++ch) {
+      const float A1 = lyf[(16 * ch + col) * 4 + k];
+      const float4 qv = reinterpret_cast<const float4*>(lq)[4 * ch + k];
+      const float4 sv = reinterpret_cast<const float4*>(ls)[4 * ch + k];
+      const float4 tv = lt[(ch * 4 + k) * 4 + (lane & 3)];
+      mf4 Cy = {qv.x, qv.y, qv.z, qv.w};
+      if (EPS) Cy += mf4{eps2x2, eps2x2, eps2x2, eps2x2};
+#pragma unroll
+      for (int tb = 0; tb < 4; ++tb) {
+        const mf4 r2 =
+            __builtin_amdgcn_mfma_f32_16x16x4f32(A1, B[tb], Cy, 0, 0, 0);
+        const float w0 = sv.x * __builtin_amdgcn_rcpf(r2[0]);
+        const float w1 = sv.y * __builtin_amdgcn_rcpf(r2[1]);
+        const float w2 = sv.z * __builtin_amdgcn_rcpf(r2[2]);
+        const float w3 = sv.w * __builtin_amdgcn_rcpf(r2[3]);
+        D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.x, w0, D2[tb], 0, 0, 0);
+        D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.y, w1, D2[tb], 0, 0, 0);
+        D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.z, w2, D2[tb], 0, 0, 0);
+        D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.w, w3, D2[tb], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) {
+      const double W = static_cast<double>(D2[tb][3]);
+      Sf[tb][0] += static_cast<double>(D2[tb][0]) + c.x * W;
+      Sf[tb][1] += static_cast<double>(D2[tb][1]) + c.y * W;
+      Sf[tb][2] += static_cast<double>(D2[tb][2]) + c.z * W;
+    }
+  }
+  // sum the four source groups (lanes col, col + 16, col + 32, col + 48)
+#pragma unroll
+  for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double v = Sf[tb][a];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      Sf[tb][a] = v;
+    }
+  // lane l is target 16 (l / 16) + l % 16 = l: take its column block k
+  double S[1][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double f = k == 0 ? Sf[0][a] : k == 1 ? Sf[1][a] : k == 2 ? Sf[2][a] : Sf[3][a];
+    S[0][a] = f + Sn[a];
+  }
+  store_cross<1>(sg, x, S, live, tgt);
+}
+
 }  // namespace
 }  // namespace lpm_amd

@@ -277,44 +277,10 @@ void run_step(lpm_ctx* ctx, const lpm_bve_fields& f, double dt, double Omega,
               const std::function<uint64_t()>& prepare, const EvalFn& eval) {
   StageBufs V = carve(ctx->rk_v, f.n_verts);
   StageBufs F = carve(ctx->rk_f, f.n_faces);
-  if (!(flags & LPM_RK4_USE_GRAPH)) {
-    issue_step(ctx, f, dt, Omega, flags, V, F, eval, s);
-    return;
-  }
-  LPM_ARG_CHECK(s != nullptr, "LPM_RK4_USE_GRAPH needs a non-default stream");
-  // no allocation may happen while the stream is being captured
-  const uint64_t serial = prepare();
-  LPM_HIP_CHECK(hipStreamSynchronize(s));
-  const std::string key =
-      graph_key(f, dt, Omega, precision, flags, s, serial);
-  auto it = ctx->graphs.find(key);
-  if (it != ctx->graphs.end() &&
-      it->second->epoch != g_workspace_epoch.load()) {
-    ctx->graphs.erase(it);
-    it = ctx->graphs.end();
-  }
-  if (it == ctx->graphs.end()) {
-    auto entry = std::make_shared<lpm_ctx::GraphEntry>();
-    const bool timing = ctx->timing;
-    ctx->timing = false;
-    LPM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      issue_step(ctx, f, dt, Omega, flags, V, F, eval, s);
-    } catch (...) {
-      ctx->timing = timing;
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    ctx->timing = timing;
-    LPM_HIP_CHECK(hipStreamEndCapture(s, &entry->graph));
-    LPM_HIP_CHECK(
-        hipGraphInstantiate(&entry->exec, entry->graph, nullptr, nullptr, 0));
-    entry->epoch = g_workspace_epoch.load();
-    it = ctx->graphs.emplace(key, entry).first;
-  }
-  LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
+  run_maybe_graph(
+      ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
+      [&]() { return graph_key(f, dt, Omega, precision, flags, s, prepare()); },
+      [&]() { issue_step(ctx, f, dt, Omega, flags, V, F, eval, s); });
 }
 
 void check_outputs(lpm_ctx* ctx, const lpm_bve_fields& f, unsigned flags,
@@ -397,6 +363,47 @@ void validate_bve_fields(const lpm_bve_fields* f) {
                 "face strides must be > 0");
 }
 
+void run_maybe_graph(lpm_ctx* ctx, bool use_graph, hipStream_t s,
+                     const std::function<std::string()>& key_fn,
+                     const std::function<void()>& issue) {
+  if (!use_graph) {
+    issue();
+    return;
+  }
+  LPM_ARG_CHECK(s != nullptr, "LPM_RK4_USE_GRAPH needs a non-default stream");
+  // the key's prepare step allocates; nothing may allocate while capturing
+  const std::string key = key_fn();
+  LPM_HIP_CHECK(hipStreamSynchronize(s));
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end() &&
+      it->second->epoch != g_workspace_epoch.load()) {
+    ctx->graphs.erase(it);
+    it = ctx->graphs.end();
+  }
+  if (it == ctx->graphs.end()) {
+    auto entry = std::make_shared<lpm_ctx::GraphEntry>();
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    LPM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      issue();
+    } catch (...) {
+      ctx->timing = timing;
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    ctx->timing = timing;
+    LPM_HIP_CHECK(hipStreamEndCapture(s, &entry->graph));
+    LPM_HIP_CHECK(
+        hipGraphInstantiate(&entry->exec, entry->graph, nullptr, nullptr, 0));
+    entry->epoch = g_workspace_epoch.load();
+    it = ctx->graphs.emplace(key, entry).first;
+  }
+  LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
+}
+
 }  // namespace lpm_amd
 
 using namespace lpm_amd;
@@ -416,12 +423,20 @@ EvalFn single_eval(lpm_ctx* ctx, const lpm_bve_fields& f, int precision,
   };
 }
 
-EvalFn ensemble_eval(lpm_ctx* ctx, const EnsembleDesc& d, int precision,
-                     unsigned flags, hipStream_t s) {
-  return [ctx, &d, precision, flags, s](const RankMap& map, CrdIn vx, CrdIn fx,
-                                        const double* fz) {
-    ensemble_velocity(ctx, d, vx, fx, fz, map, precision, flags, s);
+EvalFn ensemble_eval(lpm_ctx* ctx, const EnsembleDesc& d,
+                     const lpm_bve_fields& f, int precision, unsigned flags,
+                     hipStream_t s) {
+  return [ctx, &d, &f, precision, flags, s](const RankMap& map, CrdIn vx,
+                                            CrdIn fx, const double* fz) {
+    ensemble_velocity(ctx, d, f, vx, fx, fz, map, precision, flags, s);
   };
+}
+
+EnsembleDesc validate_ensemble(const lpm_bve_ensemble* e) {
+  LPM_ARG_CHECK(e, "null ensemble");
+  validate_bve_fields(&e->fields);
+  return validate_members(e->n_members, e->vert_offsets, e->face_offsets,
+                          e->fields.n_verts, e->fields.n_faces);
 }
 
 }  // namespace
@@ -499,15 +514,13 @@ int lpm_bve_ensemble_init_velocity(lpm_ctx* ctx, const lpm_bve_ensemble* e,
                                    void* stream) {
   try {
     LPM_ARG_CHECK(ctx, "null context");
-    LPM_ARG_CHECK(e, "null ensemble");
-    validate_bve_fields(&e->fields);
     const EnsembleDesc d = validate_ensemble(e);
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
     const lpm_bve_fields& f = e->fields;
     const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
-    ensemble_eval(ctx, d, precision, flags, s)(map, vert_crd(f), face_crd(f),
+    ensemble_eval(ctx, d, f, precision, flags, s)(map, vert_crd(f), face_crd(f),
                                                f.face_zeta);
     check_outputs(ctx, f, flags, s);
     return LPM_OK;
@@ -521,8 +534,6 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
                               unsigned flags, void* stream) {
   try {
     LPM_ARG_CHECK(ctx, "null context");
-    LPM_ARG_CHECK(e, "null ensemble");
-    validate_bve_fields(&e->fields);
     const EnsembleDesc d = validate_ensemble(e);
     check_precision(precision);
     LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
@@ -535,7 +546,7 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
       return ensemble_prepare(ctx, d, precision);
     };
     run_step(ctx, f, dt, Omega, precision, flags, s, prepare,
-             ensemble_eval(ctx, d, precision, flags, s));
+             ensemble_eval(ctx, d, f, precision, flags, s));
     check_outputs(ctx, f, flags, s);
     return LPM_OK;
   } catch (const std::exception& ex) {
@@ -549,8 +560,6 @@ int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
                               double* all_ms, void* stream) {
   try {
     LPM_ARG_CHECK(ctx, "null context");
-    LPM_ARG_CHECK(e, "null ensemble");
-    validate_bve_fields(&e->fields);
     const EnsembleDesc d = validate_ensemble(e);
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
@@ -559,7 +568,7 @@ int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
     const int family =
         precision == LPM_MIXED ? KID_BVE_ENS_MIXED : KID_BVE_ENS_F64;
     const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
-    const EvalFn ev = ensemble_eval(ctx, d, precision, flags, s);
+    const EvalFn ev = ensemble_eval(ctx, d, f, precision, flags, s);
     autotune(ctx, family, f.n_verts + f.n_faces, repeats,
              [&]() { ev(map, vert_crd(f), face_crd(f), f.face_zeta); }, s,
              best_variant, best_ms, all_ms);

@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -149,14 +150,12 @@ struct lpm_ctx {
   lpm_amd::DevBuf swe_packed;   // SWE sources (64 B each)
   lpm_amd::DevBuf swe_bounds;   // SWE tile bounding spheres
 
-  // ensemble plan and its device tables
+  // ensemble work plans (keyed by targets per work item) and scratch
   struct EnsemblePlan;
-  std::shared_ptr<EnsemblePlan> ens_plan;
+  std::map<int, std::shared_ptr<EnsemblePlan>> ens_plans;
   uint64_t ens_plan_serial = 0;
-  lpm_amd::DevBuf ens_members;  // per member: offsets, packed ranges
-  lpm_amd::DevBuf ens_items;    // int4 work items
   lpm_amd::DevBuf ens_bounds;   // int32 per member boundary
-  lpm_amd::DevBuf ens_queue;    // int32 work-queue head (+ padded total)
+  lpm_amd::DevBuf ens_queue;    // int32 work-queue head, padded total
 
   // RK4 stage buffers
   lpm_amd::DevBuf rk_v, rk_f;   // BVE vertex / face stage storage
@@ -219,26 +218,71 @@ void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
 
 // ---- RK4 (bve_rk4.hip) ----
 void validate_bve_fields(const lpm_bve_fields* f);
+// Runs issue() on stream s, or with use_graph replays the hipGraph captured
+// from it under key_fn() (which must also allocate every workspace issue()
+// uses; the graph is re-captured when a workspace was reallocated).
+void run_maybe_graph(lpm_ctx* ctx, bool use_graph, hipStream_t s,
+                     const std::function<std::string()>& key_fn,
+                     const std::function<void()>& issue);
 
-// ---- ensembles (bve_ensemble.hip) ----
-// Validated host view of an lpm_bve_ensemble.
+// ---- ensembles (ensemble.hip, bve_ensemble.hip) ----
+// Host view of an ensemble's member offsets (validated).
 struct EnsembleDesc {
-  const lpm_bve_ensemble* e = nullptr;
+  int n_members = 0;
+  const int64_t* voff = nullptr;  // host, n_members + 1
+  const int64_t* foff = nullptr;  // host, n_members + 1
+  int64_t n_verts = 0, n_faces = 0;
   int64_t max_member_faces = 0;
   int64_t total_padded_upper = 0;  // sum of member faces rounded up to 256
 };
-EnsembleDesc validate_ensemble(const lpm_bve_ensemble* e);
+EnsembleDesc validate_members(int32_t n_members, const int64_t* voff,
+                              const int64_t* foff, int64_t n_verts,
+                              int64_t n_faces);
+
+// Device member table entry.  v0/nv/f0/nf come from the host offsets; R
+// (first compacted rank of the member's undivided faces), cnt (their
+// number) and P (first packed slot, a multiple of kEnsTile) are computed on
+// the device from the face mask at every evaluation.
+constexpr int kEnsTile = 256;
+struct EnsMember {
+  int64_t v0, nv, f0, nf;
+  int32_t R, cnt, P, pad;
+};
+
+// Work plan of one offset layout for `per` targets per work item: items
+// {member, segment (0 vertices, 1 faces), block, 0} ordered by decreasing
+// member face count, and the member table.
+struct EnsLayout {
+  EnsMember* mem = nullptr;
+  const int4* items = nullptr;
+  int n_items = 0;
+  int32_t* queue = nullptr;         // work-queue head
+  int32_t* padded_total = nullptr;  // packed slots in use (device)
+  double pairs = 0;                 // sum of (member targets) x (member faces)
+  uint64_t serial = 0;              // changes whenever the plan is rebuilt
+};
+// Plan and workspaces only (no launches): safe before a stream capture.
+EnsLayout ensemble_plan(lpm_ctx* ctx, const EnsembleDesc& d, int per);
+// ensemble_plan plus the device member table from the mask (launches on s).
+EnsLayout ensemble_layout(lpm_ctx* ctx, const EnsembleDesc& d,
+                          const RankMap& map, const uint8_t* face_mask,
+                          int per, hipStream_t s);
+void ensemble_reset_queue(const EnsLayout& L, hipStream_t s);
+// Persistent grid: resident blocks of `kernel` on the device, <= n_items.
+int ensemble_grid(lpm_ctx* ctx, const void* kernel, int block, int n_items);
+
 int ens_variant_count(int family);
 const char* ens_variant_name(int family, int v);
 int ens_default_variant(int family);
-// Allocates every workspace an evaluation uses, so that a stream capture
-// that follows allocates nothing; returns the serial of the work plan.
+// BVE: allocates every workspace an evaluation uses (so that a stream
+// capture that follows allocates nothing); returns the plan serial.
 uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d, int precision);
-// Velocity of every member from positions (vx, fx) and face vorticity
-// fzeta, written to the ensemble's vert_u / face_u.
-void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d, CrdIn vx,
-                       CrdIn fx, const double* fzeta, const RankMap& map,
-                       int precision, unsigned flags, hipStream_t s);
+// BVE velocity of every member from positions (vx, fx) and face vorticity
+// fzeta, written to f.vert_u / f.face_u.
+void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
+                       const lpm_bve_fields& f, CrdIn vx, CrdIn fx,
+                       const double* fzeta, const RankMap& map, int precision,
+                       unsigned flags, hipStream_t s);
 
 // ---- sphere SWE sums (swe_sums.hip) ----
 struct SwePacked {
@@ -260,5 +304,14 @@ SwePacked swe_pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
 void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
                      int nseg, double eps, double pse_eps, int precision,
                      hipStream_t s);
+// SWE ensembles: workspaces only (returns the plan serial), and the sums of
+// every member at positions (vx, fx) from its own undivided faces.
+uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d);
+void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
+                       const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
+                       const double* fzeta, const double* fsigma,
+                       const double* farea, const RankMap& map, double eps,
+                       double pse_eps, int precision, unsigned flags,
+                       hipStream_t s);
 
 }  // namespace lpm_amd

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <functional>
+#include <sstream>
 
 #include "internal.hpp"
 
@@ -199,21 +201,24 @@ void validate(const lpm_swe_fields* f) {
 
 // Sums at positions (vx, fx) with face sources (fx, fzeta, fsigma, farea,
 // surf) into the fields' u, ddot, laps.
-void sums_eval(lpm_ctx* ctx, const lpm_swe_fields& f, const double* vx,
-               int64_t vxes, int64_t vxcs, const double* fx, int64_t fxes,
-               int64_t fxcs, const double* fzeta, const double* fsigma,
-               const double* farea, const RankMap& map, double eps,
-               double pse_eps, int precision, unsigned flags, hipStream_t s) {
-  SwePacked ps = swe_pack_sources(ctx, CrdIn{fx, fxes, fxcs}, fzeta, fsigma,
-                                  farea, f.face_surf, map, s);
+using SweEvalFn = std::function<void(const RankMap&, CrdIn vx, CrdIn fx,
+                                     const double* fzeta, const double* fsigma,
+                                     const double* farea)>;
+
+void sums_eval(lpm_ctx* ctx, const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
+               const double* fzeta, const double* fsigma, const double* farea,
+               const RankMap& map, double eps, double pse_eps, int precision,
+               unsigned flags, hipStream_t s) {
+  SwePacked ps = swe_pack_sources(ctx, fx, fzeta, fsigma, farea, f.face_surf,
+                                  map, s);
   SweTargets segs[2];
-  segs[0].base.x = CrdIn{vx, vxes, vxcs};
+  segs[0].base.x = vx;
   segs[0].base.n = f.n_verts;
   segs[0].base.out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
   segs[0].surf = f.vert_surf;
   segs[0].ddot = f.vert_ddot;
   segs[0].laps = f.vert_laps;
-  segs[1].base = face_targets(CrdIn{fx, fxes, fxcs}, map, f.face_mask,
+  segs[1].base = face_targets(fx, map, f.face_mask,
                               (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
                               CrdOut{f.face_u, f.face_u_es, f.face_u_cs},
                               nullptr);
@@ -223,10 +228,16 @@ void sums_eval(lpm_ctx* ctx, const lpm_swe_fields& f, const double* vx,
   launch_swe_sums(ctx, ps, segs, 2, eps, pse_eps, precision, s);
 }
 
+CrdIn swe_vert_crd(const lpm_swe_fields& f) {
+  return CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs};
+}
+CrdIn swe_face_crd(const lpm_swe_fields& f) {
+  return CrdIn{f.face_x, f.face_x_es, f.face_x_cs};
+}
+
 void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
-                    double Omega, double g, double eps, double pse_eps,
-                    int precision, unsigned flags, SweStage& V, SweStage& F,
-                    hipStream_t s) {
+                    double Omega, double g, unsigned flags, SweStage& V,
+                    SweStage& F, const SweEvalFn& eval, hipStream_t s) {
   const uint8_t* fskip =
       (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
   const bool compat = (flags & LPM_RK4_REFERENCE_COMPAT) != 0;
@@ -234,8 +245,7 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
   const double coef[3] = {0.5, 0.5, 1.0};
   for (int stage = 0; stage < 4; ++stage) {
     if (stage > 0)
-      sums_eval(ctx, f, V.xw, 1, V.n, F.xw, 1, F.n, F.zw, F.sw, F.hw, map, eps,
-                pse_eps, precision, flags, s);
+      eval(map, CrdIn{V.xw, 1, V.n}, CrdIn{F.xw, 1, F.n}, F.zw, F.sw, F.hw);
     for (int kind = 0; kind < 2; ++kind) {
       const bool face = kind == 1;
       SweStage& B = face ? F : V;
@@ -323,9 +333,73 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
     swe_update_kernel<<<nblk(n), 256, 0, s>>>(u);
     LPM_HIP_CHECK(hipGetLastError());
   }
-  sums_eval(ctx, f, f.vert_x, f.vert_x_es, f.vert_x_cs, f.face_x, f.face_x_es,
-            f.face_x_cs, f.face_zeta, f.face_sigma, f.face_area, map, eps,
-            pse_eps, precision, flags, s);
+  eval(map, swe_vert_crd(f), swe_face_crd(f), f.face_zeta, f.face_sigma,
+       f.face_area);
+}
+
+std::string swe_graph_key(const lpm_swe_fields& f, double dt, double Omega,
+                          double g, double eps, double pse_eps, int precision,
+                          unsigned flags, hipStream_t s, uint64_t serial) {
+  std::ostringstream k;
+  k << "swe|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
+    << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_sigma << '|'
+    << f.vert_h << '|' << f.vert_u << '|' << f.vert_u_es << '|'
+    << f.vert_u_cs << '|' << f.vert_ddot << '|' << f.vert_laps << '|'
+    << f.vert_surf << '|' << f.n_faces << '|' << f.face_x << '|'
+    << f.face_x_es << '|' << f.face_x_cs << '|' << f.face_zeta << '|'
+    << f.face_sigma << '|' << f.face_area << '|' << f.face_mass << '|'
+    << f.face_h << '|' << f.face_u << '|' << f.face_u_es << '|'
+    << f.face_u_cs << '|' << f.face_ddot << '|' << f.face_laps << '|'
+    << f.face_surf << '|' << static_cast<const void*>(f.face_mask) << '|';
+  k.precision(17);
+  k << dt << '|' << Omega << '|' << g << '|' << eps << '|' << pse_eps << '|'
+    << precision << '|' << flags << '|' << s << '|' << serial;
+  return k.str();
+}
+
+void check_step_args(lpm_ctx* ctx, int precision, double dt, double Omega,
+                     double g, double eps, double pse_eps) {
+  LPM_ARG_CHECK(ctx, "null context");
+  LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                "unknown precision");
+  LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega) && std::isfinite(g),
+                "dt, Omega, g must be finite");
+  LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
+}
+
+void check_swe_outputs(lpm_ctx* ctx, const lpm_swe_fields& f, unsigned flags,
+                       hipStream_t s) {
+  if (!(flags & LPM_FLAG_CHECK_FINITE)) return;
+  check_finite(ctx, f.vert_u, f.n_verts, f.vert_u_es, f.vert_u_cs, 3, s);
+  check_finite(ctx, f.face_u, f.n_faces, f.face_u_es, f.face_u_cs, 3, s);
+}
+
+SweEvalFn single_swe_eval(lpm_ctx* ctx, const lpm_swe_fields& f, double eps,
+                          double pse_eps, int precision, unsigned flags,
+                          hipStream_t s) {
+  return [=, &f](const RankMap& map, CrdIn vx, CrdIn fx, const double* fz,
+                 const double* fsig, const double* fa) {
+    sums_eval(ctx, f, vx, fx, fz, fsig, fa, map, eps, pse_eps, precision,
+              flags, s);
+  };
+}
+
+SweEvalFn ensemble_swe_eval(lpm_ctx* ctx, const EnsembleDesc& d,
+                            const lpm_swe_fields& f, double eps,
+                            double pse_eps, int precision, unsigned flags,
+                            hipStream_t s) {
+  return [=, &d, &f](const RankMap& map, CrdIn vx, CrdIn fx, const double* fz,
+                     const double* fsig, const double* fa) {
+    swe_ensemble_sums(ctx, d, f, vx, fx, fz, fsig, fa, map, eps, pse_eps,
+                      precision, flags, s);
+  };
+}
+
+EnsembleDesc validate_swe_ensemble(const lpm_swe_ensemble* e) {
+  LPM_ARG_CHECK(e, "null ensemble");
+  validate(&e->fields);
+  return validate_members(e->n_members, e->vert_offsets, e->face_offsets,
+                          e->fields.n_verts, e->fields.n_faces);
 }
 
 template <typename F>
@@ -418,9 +492,9 @@ int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
-    sums_eval(ctx, *f, f->vert_x, f->vert_x_es, f->vert_x_cs, f->face_x,
-              f->face_x_es, f->face_x_cs, f->face_zeta, f->face_sigma,
-              f->face_area, map, eps, pse_eps, precision, flags, s);
+    sums_eval(ctx, *f, swe_vert_crd(*f), swe_face_crd(*f), f->face_zeta,
+              f->face_sigma, f->face_area, map, eps, pse_eps, precision, flags,
+              s);
   });
 }
 
@@ -428,25 +502,71 @@ int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
                      double Omega, double g, double eps, double pse_eps,
                      int precision, unsigned flags, void* stream) {
   return guarded(ctx, [&] {
-    LPM_ARG_CHECK(ctx, "null context");
+    check_step_args(ctx, precision, dt, Omega, g, eps, pse_eps);
     validate(f);
-    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
-                  "unknown precision");
-    LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega) && std::isfinite(g),
-                  "dt, Omega, g must be finite");
-    LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
-    LPM_ARG_CHECK(!(flags & LPM_RK4_USE_GRAPH),
-                  "graph capture is available for the BVE stepper only");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
     SweStage V = carve(ctx->swe_v, f->n_verts);
     SweStage F = carve(ctx->swe_f, f->n_faces);
-    issue_swe_step(ctx, *f, dt, Omega, g, eps, pse_eps, precision, flags, V, F,
-                   s);
-    if (flags & LPM_FLAG_CHECK_FINITE) {
-      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
-      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
-    }
+    const SweEvalFn ev =
+        single_swe_eval(ctx, *f, eps, pse_eps, precision, flags, s);
+    run_maybe_graph(
+        ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
+        [&]() {
+          const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+          swe_pack_sources(ctx, swe_face_crd(*f), f->face_zeta, f->face_sigma,
+                           f->face_area, f->face_surf, map, s);
+          return swe_graph_key(*f, dt, Omega, g, eps, pse_eps, precision,
+                               flags, s, 0);
+        },
+        [&]() { issue_swe_step(ctx, *f, dt, Omega, g, flags, V, F, ev, s); });
+    check_swe_outputs(ctx, *f, flags, s);
+  });
+}
+
+int lpm_swe_ensemble_init_sums(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                               double eps, double pse_eps, int precision,
+                               unsigned flags, void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    const EnsembleDesc d = validate_swe_ensemble(e);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
+    LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_swe_fields& f = e->fields;
+    const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+    swe_ensemble_sums(ctx, d, f, swe_vert_crd(f), swe_face_crd(f),
+                      f.face_zeta, f.face_sigma, f.face_area, map, eps,
+                      pse_eps, precision, flags, s);
+    check_swe_outputs(ctx, f, flags, s);
+  });
+}
+
+int lpm_swe_ensemble_rk4_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                              double dt, double Omega, double g, double eps,
+                              double pse_eps, int precision, unsigned flags,
+                              void* stream) {
+  return guarded(ctx, [&] {
+    check_step_args(ctx, precision, dt, Omega, g, eps, pse_eps);
+    const EnsembleDesc d = validate_swe_ensemble(e);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_swe_fields& f = e->fields;
+    SweStage V = carve(ctx->swe_v, f.n_verts);
+    SweStage F = carve(ctx->swe_f, f.n_faces);
+    const SweEvalFn ev =
+        ensemble_swe_eval(ctx, d, f, eps, pse_eps, precision, flags, s);
+    run_maybe_graph(
+        ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
+        [&]() {
+          build_rank(ctx, f.face_mask, f.n_faces, s);
+          return swe_graph_key(f, dt, Omega, g, eps, pse_eps, precision, flags,
+                               s, swe_ensemble_prepare(ctx, d));
+        },
+        [&]() { issue_swe_step(ctx, f, dt, Omega, g, flags, V, F, ev, s); });
+    check_swe_outputs(ctx, f, flags, s);
   });
 }
 

@@ -128,7 +128,24 @@ struct SweSeg {
   int64_t oes, ocs;
   double* out_ddot;
   double* out_laps;
+  int64_t slot_base;  // slot c is target index[c] or slot_base + c
+  int32_t rank_base;  // self = self_rank[i] - rank_base
 };
+
+// Target of slot c and its own source (-1: none) as in the BVE kernels.
+__device__ __forceinline__ int64_t swe_target(const SweSeg& sg, int64_t c,
+                                              bool in) {
+  return in ? (sg.index ? sg.index[c] : sg.slot_base + c) : sg.slot_base;
+}
+__device__ __forceinline__ int swe_self(const SweSeg& sg, int64_t c,
+                                        int64_t i, bool in, int skip_self) {
+  if (!in || !skip_self) return -1;
+  if (sg.self_rank) {
+    const int r = sg.self_rank[i];
+    return r < 0 ? -1 : r - sg.rank_base;
+  }
+  return sg.self_identity ? static_cast<int>(c) : -1;
+}
 
 // Accumulators of one target (see the file comment).
 template <typename T>
@@ -251,28 +268,30 @@ __device__ __forceinline__ void swe_finalize(const SweAcc<double>& A, const doub
 // ----------------------------------------------------------- FP64 kernel --
 // One target per lane; sources staged through LDS.  PSE terms are evaluated
 // only for tiles near some target of the wave.
+// Parameters shared by every target of a launch.
+struct SweParams {
+  double eps2, inv_pse_e2, pse_cut2;
+  int do_pse, skip_self;
+};
+
+// Block `blk` of segment `sg` against nsrc packed sources (bounds: their
+// tile bounding spheres).  Every thread of the block calls it.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_f64_kernel(
+__device__ __forceinline__ void swe_f64_block(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
-    const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, double eps2,
-    double inv_pse_e2, double pse_cut2, int do_pse, int skip_self) {
-  __shared__ SweSrc tile[kSweTile];
-  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
-  const SweSeg& sg = first ? s0 : s1;
-  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  const int nsrc = *count;
+    int nsrc, const SweSeg& sg, int blk, const SweParams& P, SweSrc* tile) {
+  const double eps2 = P.eps2, inv_pse_e2 = P.inv_pse_e2;
+  const double pse_cut2 = P.pse_cut2;
+  const int do_pse = P.do_pse;
   const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
   const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
   if (static_cast<int64_t>(blk) * BLOCK >= n_live) return;
   const bool in = c < n_live;
-  const int64_t i = in ? (sg.index ? sg.index[c] : c) : 0;
+  const int64_t i = swe_target(sg, c, in);
   const double* px = sg.x + i * sg.es;
   const double x[3] = {px[0], px[sg.cs], px[2 * sg.cs]};
   const double s_i = (do_pse && sg.surf) ? sg.surf[i] : 0.0;
-  int self = -1;
-  if (in && skip_self)
-    self = sg.self_rank ? sg.self_rank[i]
-                        : (sg.self_identity ? static_cast<int>(c) : -1);
+  const int self = swe_self(sg, c, i, in, P.skip_self);
 
   SweAcc<double> acc;
   acc.zero();
@@ -335,37 +354,45 @@ __global__ __launch_bounds__(BLOCK) void swe_f64_kernel(
   if (sg.out_laps && do_pse) sg.out_laps[i] = lap;
 }
 
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void swe_f64_kernel(
+    const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
+    const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
+  __shared__ SweSrc tile[kSweTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SweSeg& sg = first ? s0 : s1;
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  swe_f64_block<BLOCK>(src, bounds, *count, sg, blk, P, tile);
+}
+
 // ---------------------------------------------------------- mixed kernel --
 // FP32 pair arithmetic on tile-relative coordinates (d formed from offsets
 // to the tile's FP64 centre, delta = |d|^2 / 2); FP32 partial sums per
 // source tile are carried into FP64 accumulators.
+struct F4x2 {
+  float4 p;  // y - c (3), a
+  float4 q;  // b, A, 0, 0
+};
+
+// As swe_f64_block with FP32 pair arithmetic; tile64 is the FP64 copy of
+// the LDS tile for the PSE terms.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_mixed_kernel(
+__device__ __forceinline__ void swe_mixed_block(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
-    const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, double eps2,
-    double inv_pse_e2, double pse_cut2, int do_pse, int skip_self) {
-  struct F4x2 {
-    float4 p;  // y - c (3), a
-    float4 q;  // b, A, 0, 0
-  };
-  __shared__ F4x2 tile[kSweTile];
-  __shared__ SweSrc tile64[kSweTile];  // FP64 copy for the PSE terms
-  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
-  const SweSeg& sg = first ? s0 : s1;
-  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  const int nsrc = *count;
+    int nsrc, const SweSeg& sg, int blk, const SweParams& P, F4x2* tile,
+    SweSrc* tile64) {
+  const double eps2 = P.eps2, inv_pse_e2 = P.inv_pse_e2;
+  const double pse_cut2 = P.pse_cut2;
+  const int do_pse = P.do_pse;
   const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
   const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
   if (static_cast<int64_t>(blk) * BLOCK >= n_live) return;
   const bool in = c < n_live;
-  const int64_t i = in ? (sg.index ? sg.index[c] : c) : 0;
+  const int64_t i = swe_target(sg, c, in);
   const double* px = sg.x + i * sg.es;
   const double x[3] = {px[0], px[sg.cs], px[2 * sg.cs]};
   const double s_i = (do_pse && sg.surf) ? sg.surf[i] : 0.0;
-  int self = -1;
-  if (in && skip_self)
-    self = sg.self_rank ? sg.self_rank[i]
-                        : (sg.self_identity ? static_cast<int>(c) : -1);
+  const int self = swe_self(sg, c, i, in, P.skip_self);
   const float e2f = static_cast<float>(eps2);
 
   SweAcc<double> acc;
@@ -464,6 +491,154 @@ __global__ __launch_bounds__(BLOCK) void swe_mixed_kernel(
   if (sg.out_laps && do_pse) sg.out_laps[i] = lap;
 }
 
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void swe_mixed_kernel(
+    const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
+    const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
+  __shared__ F4x2 tile[kSweTile];
+  __shared__ SweSrc tile64[kSweTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SweSeg& sg = first ? s0 : s1;
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  swe_mixed_block<BLOCK>(src, bounds, *count, sg, blk, P, tile, tile64);
+}
+
+// ------------------------------------------------------------ ensembles --
+// grid (ceil(max member faces / 256), n_members): member m's c-th undivided
+// face at packed slot P_m + c; zero sources pad the member's last tile.
+__global__ __launch_bounds__(256) void swe_ens_pack_kernel(
+    const double* __restrict__ sx, int64_t es, int64_t cs,
+    const double* __restrict__ zeta, const double* __restrict__ sigma,
+    const double* __restrict__ area, const double* __restrict__ surf,
+    const int32_t* __restrict__ inv, const EnsMember* __restrict__ mem,
+    SweSrc* __restrict__ out) {
+  const EnsMember M = mem[blockIdx.y];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int padded = (M.cnt + kSweTile - 1) / kSweTile * kSweTile;
+  if (c >= padded) return;
+  SweSrc v;
+  v.p = make_double4(0.0, 0.0, 0.0, 0.0);
+  v.q = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (c < M.cnt) {
+    const int64_t r = static_cast<int64_t>(M.R) + c;
+    const int64_t i = inv ? inv[r] : r;
+    const double* p = sx + i * es;
+    const double A = area[i];
+    v.p = make_double4(p[0], p[cs], p[2 * cs], zeta[i] * A * kInv4Pi);
+    v.q = make_double4(sigma[i] * A * kInv4Pi, A, surf ? surf[i] : 0.0, 0.0);
+  }
+  out[M.P + c] = v;
+}
+
+// grid (ceil(max member faces / 256), n_members): bounding sphere of tile t
+// of member m over its real sources only.
+__global__ __launch_bounds__(kSweTile) void swe_ens_bounds_kernel(
+    const SweSrc* __restrict__ src, const EnsMember* __restrict__ mem,
+    double4* __restrict__ bounds) {
+  __shared__ double red[kSweTile / 64];
+  const EnsMember M = mem[blockIdx.y];
+  const int j0 = blockIdx.x * kSweTile;
+  if (j0 >= M.cnt) return;
+  const double4 c = src[M.P + j0].p;
+  const int j = j0 + threadIdx.x;
+  double r2 = 0.0;
+  if (j < M.cnt) {
+    const double4 y = src[M.P + j].p;
+    const double a = y.x - c.x, b = y.y - c.y, e = y.z - c.z;
+    r2 = a * a + b * b + e * e;
+  }
+  for (int off = 32; off > 0; off >>= 1) r2 = fmax(r2, __shfl_down(r2, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kSweTile / 64; ++w) m = fmax(m, red[w]);
+    bounds[(M.P + j0) / kSweTile] = make_double4(c.x, c.y, c.z, m);
+  }
+}
+
+struct SweEnsArgs {
+  const EnsMember* mem;
+  const int4* items;
+  int n_items;
+  int32_t* queue;
+  SweSeg vert;  // arrays of the vertex targets (x, surf, outputs)
+  SweSeg face;  // arrays of the face targets
+  const int32_t* rank;
+  const int32_t* inv;
+  int skip_masked;
+  SweParams P;
+};
+
+__device__ __forceinline__ SweSeg swe_ens_segment(const SweEnsArgs& A,
+                                                  const EnsMember& M, int seg) {
+  if (seg == 0) {
+    SweSeg sg = A.vert;
+    sg.n = M.nv;
+    sg.slot_base = M.v0;
+    return sg;
+  }
+  SweSeg sg = A.face;
+  if (A.skip_masked && A.inv) {
+    sg.index = A.inv + M.R;
+    sg.n = M.cnt;
+    sg.self_identity = 1;
+  } else {
+    sg.n = M.nf;
+    sg.slot_base = M.f0;
+    if (A.rank) {
+      sg.self_rank = A.rank;
+      sg.rank_base = M.R;
+    } else {
+      sg.self_identity = 1;
+    }
+  }
+  return sg;
+}
+
+__device__ __forceinline__ int swe_ens_next(const SweEnsArgs& A, int* slot) {
+  if (threadIdx.x == 0) *slot = atomicAdd(A.queue, 1);
+  __syncthreads();
+  const int it = *slot;
+  __syncthreads();
+  return it;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void swe_ens_f64_kernel(
+    const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
+    SweEnsArgs A) {
+  __shared__ SweSrc tile[kSweTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = swe_ens_next(A, &slot);
+    if (it >= A.n_items) break;
+    const int4 w = A.items[it];
+    const EnsMember M = A.mem[w.x];
+    const SweSeg sg = swe_ens_segment(A, M, w.y);
+    swe_f64_block<BLOCK>(src + M.P, bounds + M.P / kSweTile, M.cnt, sg, w.z,
+                         A.P, tile);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void swe_ens_mixed_kernel(
+    const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
+    SweEnsArgs A) {
+  __shared__ F4x2 tile[kSweTile];
+  __shared__ SweSrc tile64[kSweTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = swe_ens_next(A, &slot);
+    if (it >= A.n_items) break;
+    const int4 w = A.items[it];
+    const EnsMember M = A.mem[w.x];
+    const SweSeg sg = swe_ens_segment(A, M, w.y);
+    swe_mixed_block<BLOCK>(src + M.P, bounds + M.P / kSweTile, M.cnt, sg, w.z,
+                           A.P, tile, tile64);
+  }
+}
+
 SweSeg to_seg(const SweTargets& t, int per_block) {
   SweSeg a{};
   a.x = t.base.x.base;
@@ -527,18 +702,98 @@ void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
   // SphereFaceSums skips j == i only for the singular kernel (eps == 0)
   const int skip_self = eps == 0.0 ? 1 : 0;
   double pairs = static_cast<double>(t0.base.n + t1.base.n) * src.capacity;
+  const SweParams P{eps * eps, inv_e2, cut2, do_pse, skip_self};
   if (precision == LPM_FP64) {
     TimedLaunch tl(ctx, KID_SWE_F64, s, pairs);
     hipLaunchKernelGGL((swe_f64_kernel<kBlock>), dim3(grid), dim3(kBlock), 0, s,
                        static_cast<const SweSrc*>(src.data), src.bounds,
-                       src.count, a0, a1, eps * eps, inv_e2, cut2, do_pse,
-                       skip_self);
+                       src.count, a0, a1, P);
   } else {
     TimedLaunch tl(ctx, KID_SWE_MIXED, s, pairs);
     hipLaunchKernelGGL((swe_mixed_kernel<kBlock>), dim3(grid), dim3(kBlock), 0,
                        s, static_cast<const SweSrc*>(src.data), src.bounds,
-                       src.count, a0, a1, eps * eps, inv_e2, cut2, do_pse,
-                       skip_self);
+                       src.count, a0, a1, P);
+  }
+  LPM_HIP_CHECK(hipGetLastError());
+}
+
+uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d) {
+  const EnsLayout L = ensemble_plan(ctx, d, 256);
+  const int64_t cap = std::max<int64_t>(d.total_padded_upper, kSweTile);
+  ctx->swe_packed.ensure(sizeof(SweSrc) * cap);
+  ctx->swe_bounds.ensure(sizeof(double4) * (cap / kSweTile));
+  return L.serial;
+}
+
+void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
+                       const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
+                       const double* fzeta, const double* fsigma,
+                       const double* farea, const RankMap& map, double eps,
+                       double pse_eps, int precision, unsigned flags,
+                       hipStream_t s) {
+  constexpr int kBlock = 256;
+  swe_ensemble_prepare(ctx, d);
+  const EnsLayout L =
+      ensemble_layout(ctx, d, map, f.face_mask, kBlock, s);
+  SweSrc* src = ctx->swe_packed.as<SweSrc>();
+  double4* bounds = ctx->swe_bounds.as<double4>();
+  const dim3 grid2(static_cast<unsigned>((d.max_member_faces + 255) / 256),
+                   static_cast<unsigned>(d.n_members));
+  swe_ens_pack_kernel<<<grid2, 256, 0, s>>>(fx.base, fx.es, fx.cs, fzeta,
+                                             fsigma, farea, f.face_surf,
+                                             map.inv, L.mem, src);
+  LPM_HIP_CHECK(hipGetLastError());
+  swe_ens_bounds_kernel<<<grid2, kSweTile, 0, s>>>(src, L.mem, bounds);
+  LPM_HIP_CHECK(hipGetLastError());
+  if (L.n_items == 0) return;
+
+  SweEnsArgs A{};
+  A.mem = L.mem;
+  A.items = L.items;
+  A.n_items = L.n_items;
+  A.queue = L.queue;
+  A.vert.x = vx.base;
+  A.vert.es = vx.es;
+  A.vert.cs = vx.cs;
+  A.vert.surf = f.vert_surf;
+  A.vert.out_u = f.vert_u;
+  A.vert.oes = f.vert_u_es;
+  A.vert.ocs = f.vert_u_cs;
+  A.vert.out_ddot = f.vert_ddot;
+  A.vert.out_laps = f.vert_laps;
+  A.face.x = fx.base;
+  A.face.es = fx.es;
+  A.face.cs = fx.cs;
+  A.face.surf = f.face_surf;
+  A.face.out_u = f.face_u;
+  A.face.oes = f.face_u_es;
+  A.face.ocs = f.face_u_cs;
+  A.face.out_ddot = f.face_ddot;
+  A.face.out_laps = f.face_laps;
+  A.rank = map.rank;
+  A.inv = map.inv;
+  A.skip_masked = (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? 1 : 0;
+  const int do_pse = pse_eps > 0 ? 1 : 0;
+  A.P = SweParams{eps * eps, do_pse ? 1.0 / (pse_eps * pse_eps) : 0.0,
+                  do_pse ? kPseCut * pse_eps * pse_eps : 0.0, do_pse,
+                  eps == 0.0 ? 1 : 0};
+  ensemble_reset_queue(L, s);
+  if (precision == LPM_FP64) {
+    const int grid = ensemble_grid(
+        ctx, reinterpret_cast<const void*>(&swe_ens_f64_kernel<kBlock>),
+        kBlock, L.n_items);
+    TimedLaunch tl(ctx, KID_SWE_F64, s, L.pairs);
+    hipLaunchKernelGGL((swe_ens_f64_kernel<kBlock>), dim3(grid), dim3(kBlock),
+                       0, s, static_cast<const SweSrc*>(src),
+                       static_cast<const double4*>(bounds), A);
+  } else {
+    const int grid = ensemble_grid(
+        ctx, reinterpret_cast<const void*>(&swe_ens_mixed_kernel<kBlock>),
+        kBlock, L.n_items);
+    TimedLaunch tl(ctx, KID_SWE_MIXED, s, L.pairs);
+    hipLaunchKernelGGL((swe_ens_mixed_kernel<kBlock>), dim3(grid),
+                       dim3(kBlock), 0, s, static_cast<const SweSrc*>(src),
+                       static_cast<const double4*>(bounds), A);
   }
   LPM_HIP_CHECK(hipGetLastError());
 }

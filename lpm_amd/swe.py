@@ -8,6 +8,8 @@ sums, tendencies and stage updates run in the library's HIP kernels.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -122,7 +124,8 @@ class SWERK4:
     """Sphere RK4 for SWE (4 direct-sum evaluations per step)."""
 
     def __init__(self, dt: float, swe: SWE, precision="fp64",
-                 reference_compat=True, leaf_only=False):
+                 reference_compat=True, leaf_only=False, use_graph=False,
+                 stream=None):
         self.dt = float(dt)
         self.t_idx = 0
         self.precision = precision
@@ -131,12 +134,114 @@ class SWERK4:
             flags |= _abi.LPM_RK4_REFERENCE_COMPAT
         if leaf_only:
             flags |= _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+        if use_graph:
+            flags |= _abi.LPM_RK4_USE_GRAPH
         self.flags = flags
+        self.stream = stream
         self._fields = None
 
     def advance_timestep_impl(self, swe: SWE):
         if self._fields is None:
             self._fields = swe.fields()
         swe.ctx.swe_rk4_step(self._fields, self.dt, swe.Omega, swe.g, swe.eps,
-                             swe.pse_eps, self.precision, self.flags)
+                             swe.pse_eps, self.precision, self.flags,
+                             self.stream)
+        self.t_idx += 1
+
+
+class SWEEnsemble(SWE):
+    """Independent sphere SWE problems stored back to back (same field names
+    as SWE, concatenated), evaluated by one persistent launch per direct sum
+    (``lpm_swe_ensemble_*``).  Member m owns vertices
+    ``vert_offsets[m]:vert_offsets[m+1]`` and faces
+    ``face_offsets[m]:face_offsets[m+1]``.  No reference counterpart."""
+
+    def __init__(self, meshes, Omega=2 * np.pi, g=1.0, ctx: Context = None):
+        self.meshes = list(meshes)
+        assert self.meshes, "an ensemble needs at least one member"
+        ms = self.meshes
+        # concatenated particles (parents and corner indices stay per member)
+        cat = SphereMesh(
+            seed=-1, depth=-1,
+            vert_x=np.concatenate([m.vert_x for m in ms], axis=1),
+            face_x=np.concatenate([m.face_x for m in ms], axis=1),
+            face_area=np.concatenate([m.face_area for m in ms]),
+            face_mask=np.concatenate([m.face_mask for m in ms]),
+            face_parent=None, face_verts=None,
+            n_leaves=sum(m.n_leaves for m in ms),
+            n_edges=sum(m.n_edges for m in ms))
+        super().__init__(None, None, Omega, g, ctx, mesh=cat)
+        self.vert_offsets = np.concatenate(
+            [[0], np.cumsum([m.n_verts for m in self.meshes])]).astype(np.int64)
+        self.face_offsets = np.concatenate(
+            [[0], np.cumsum([m.n_faces for m in self.meshes])]).astype(np.int64)
+        self._c_voff = (ctypes.c_int64 * len(self.vert_offsets))(
+            *self.vert_offsets.tolist())
+        self._c_foff = (ctypes.c_int64 * len(self.face_offsets))(
+            *self.face_offsets.tolist())
+
+    @property
+    def n_members(self) -> int:
+        return len(self.meshes)
+
+    def vert_slice(self, m) -> slice:
+        return slice(int(self.vert_offsets[m]), int(self.vert_offsets[m + 1]))
+
+    def face_slice(self, m) -> slice:
+        return slice(int(self.face_offsets[m]), int(self.face_offsets[m + 1]))
+
+    def useful_pairs(self) -> int:
+        return sum(m.useful_pairs() for m in self.meshes)
+
+    def _per_member(self, fns, x, slicer):
+        out = np.empty(x.shape[1])
+        for m, fn in enumerate(fns):
+            sl = slicer(m)
+            out[sl] = fn(x[:, sl])
+        return out
+
+    def init_surface(self, fns):
+        """Surface of member m = fns[m](x); zero bottom (depth = surface)."""
+        sv = self._per_member(fns, self.mesh.vert_x, self.vert_slice)
+        sf = self._per_member(fns, self.mesh.face_x, self.face_slice)
+        self.surf_passive.copy_(torch.from_numpy(sv))
+        self.depth_passive.copy_(torch.from_numpy(sv))
+        self.surf_active.copy_(torch.from_numpy(sf))
+        self.depth_active.copy_(torch.from_numpy(sf))
+        self.mass_active.copy_(torch.from_numpy(sf * self.mesh.face_area))
+
+    def init_vorticity(self, fns):
+        self.rel_vort_passive.copy_(torch.from_numpy(
+            self._per_member(fns, self.mesh.vert_x, self.vert_slice)))
+        self.rel_vort_active.copy_(torch.from_numpy(
+            self._per_member(fns, self.mesh.face_x, self.face_slice)))
+
+    def init_divergence(self, fns):
+        self.div_passive.copy_(torch.from_numpy(
+            self._per_member(fns, self.mesh.vert_x, self.vert_slice)))
+        self.div_active.copy_(torch.from_numpy(
+            self._per_member(fns, self.mesh.face_x, self.face_slice)))
+
+    def descriptor(self) -> _abi.SweEnsemble:
+        e = _abi.SweEnsemble()
+        e.n_members = self.n_members
+        e.vert_offsets = ctypes.cast(self._c_voff, _abi.c_i64p)
+        e.face_offsets = ctypes.cast(self._c_foff, _abi.c_i64p)
+        e.fields = self.fields()
+        return e
+
+    def init_direct_sums(self, precision="fp64", flags=0):
+        self.ctx.swe_ensemble_init_sums(self.descriptor(), self.eps,
+                                        self.pse_eps, precision, flags)
+
+
+class SWEEnsembleRK4(SWERK4):
+    """RK4 of every member of an SWEEnsemble (flags as SWERK4)."""
+
+    def advance_timestep_impl(self, swe: SWEEnsemble):
+        if self._fields is None:
+            self._fields = swe.descriptor()
+        swe.ctx.swe_ensemble_rk4_step(self._fields, self.dt, swe.Omega, swe.g,
+                                      swe.eps, swe.pse_eps, self.precision,
+                                      self.flags, self.stream)
         self.t_idx += 1
