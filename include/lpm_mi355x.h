@@ -222,6 +222,63 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
                               double dt, double Omega, int precision,
                               unsigned flags, void* stream);
 
+/* ---------------------------------------------- Incompressible2D ---- */
+/* Fused velocity and stream function of 2-D incompressible flow on the
+ * sphere, with kernel smoothing eps (arg = 1 - x.y + eps^2, Gamma = zeta A):
+ *   u(x_i)   = sum_j (x_i cross y_j) Gamma_j / (-4 pi arg),
+ *   psi(x_i) = sum_j -log(arg) Gamma_j / (4 pi),
+ * over the undivided faces.  Replaces Incompressible2DPassiveSums
+ * (collocated = 0) and Incompressible2DActiveSums (collocated = 1) with
+ * Incompressible2DReducer and Incompressible2DKernels<SphereGeometry>::
+ * kernel_vals (src/lpm_incompressible2d_kernels.hpp:35-47, 96-247).
+ * Collocated sums skip j == i only when eps == 0, as the reference does
+ * (:242).  FP64 only.  Arguments as lpm_bve_velocity; out_u or out_psi may
+ * be NULL. */
+int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                  int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                  int64_t src_es, int64_t src_cs, const double* src_zeta,
+                  const double* src_area, const uint8_t* src_mask,
+                  int64_t n_src, int collocated, double eps, unsigned flags,
+                  double* out_u, int64_t out_es, int64_t out_cs,
+                  double* out_psi, void* stream);
+
+/* Fields of Incompressible2D<Seed> (src/lpm_incompressible2d.hpp) the
+ * stepper reads and writes. */
+typedef struct lpm_ic2d_fields {
+  int64_t n_verts;
+  double* vert_x;
+  int64_t vert_x_es, vert_x_cs;
+  double* vert_zeta;
+  double* vert_u;
+  int64_t vert_u_es, vert_u_cs;
+  double* vert_psi;
+  int64_t n_faces;
+  double* face_x;
+  int64_t face_x_es, face_x_cs;
+  double* face_zeta;
+  double* face_u;
+  int64_t face_u_es, face_u_cs;
+  double* face_psi;
+  const double* face_area;
+  const uint8_t* face_mask;
+} lpm_ic2d_fields;
+
+/* Velocity and stream function at the current state
+ * (Incompressible2D::init_direct_sums,
+ * src/lpm_incompressible2d_impl.hpp:235-254). */
+int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
+                              double eps, unsigned flags, void* stream);
+
+/* One step of Incompressible2DRK2 (Heun's method,
+ * src/lpm_incompressible2d_rk2_impl.hpp:75-172) with
+ * Incompressible2DTendencies (dzeta/dt = -2 Omega u_z, CoriolisSphere;
+ * src/lpm_incompressible2d_kernels.hpp:256-277).  On entry u must be
+ * current; on exit u and psi are at the new state.  Flags:
+ * LPM_FLAG_SKIP_MASKED_TARGETS, LPM_RK4_RENORMALIZE (stage and final
+ * positions projected onto the sphere), LPM_RK4_USE_GRAPH. */
+int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
+                      double Omega, double eps, unsigned flags, void* stream);
+
 /* --------------------------------------------------------- SWE sums ---- */
 /* Sphere shallow-water direct sums at the targets: velocity from relative
  * vorticity zeta and divergence sigma,
@@ -364,6 +421,7 @@ int lpm_tune_load(lpm_ctx* ctx, const char* path);
 #define LPM_KERNEL_SWE_MIXED 4
 #define LPM_KERNEL_BVE_ENSEMBLE_F64 5
 #define LPM_KERNEL_BVE_ENSEMBLE_MIXED 6
+#define LPM_KERNEL_IC2D 7
 int lpm_timing_enable(lpm_ctx* ctx, int enable);
 int lpm_timing_query(lpm_ctx* ctx, int kernel_id, double* total_ms,
                      int64_t* launches, double* pairs);

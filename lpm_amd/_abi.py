@@ -38,6 +38,7 @@ LPM_KERNEL_SWE_F64 = 3
 LPM_KERNEL_SWE_MIXED = 4
 LPM_KERNEL_BVE_ENSEMBLE_F64 = 5
 LPM_KERNEL_BVE_ENSEMBLE_MIXED = 6
+LPM_KERNEL_IC2D = 7
 
 c_int = ctypes.c_int
 c_uint = ctypes.c_uint
@@ -67,6 +68,19 @@ class BveFields(ctypes.Structure):
         ("face_x_cs", c_i64), ("face_zeta", c_vp), ("face_u", c_vp),
         ("face_u_es", c_i64), ("face_u_cs", c_i64), ("face_area", c_vp),
         ("face_mask", c_vp),
+    ]
+
+
+class Ic2dFields(ctypes.Structure):
+    """struct lpm_ic2d_fields."""
+    _fields_ = [
+        ("n_verts", c_i64), ("vert_x", c_vp), ("vert_x_es", c_i64),
+        ("vert_x_cs", c_i64), ("vert_zeta", c_vp), ("vert_u", c_vp),
+        ("vert_u_es", c_i64), ("vert_u_cs", c_i64), ("vert_psi", c_vp),
+        ("n_faces", c_i64), ("face_x", c_vp), ("face_x_es", c_i64),
+        ("face_x_cs", c_i64), ("face_zeta", c_vp), ("face_u", c_vp),
+        ("face_u_es", c_i64), ("face_u_cs", c_i64), ("face_psi", c_vp),
+        ("face_area", c_vp), ("face_mask", c_vp),
     ]
 
 
@@ -147,6 +161,13 @@ _SIGS = {
     "lpm_swe_ensemble_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
                                           c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                           c_int, c_uint, c_vp]),
+    "lpm_ic2d_sums": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
+                              c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_dbl,
+                              c_uint, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "lpm_ic2d_init_direct_sums": (c_int, [c_vp, ctypes.POINTER(Ic2dFields),
+                                          c_dbl, c_uint, c_vp]),
+    "lpm_ic2d_rk2_step": (c_int, [c_vp, ctypes.POINTER(Ic2dFields), c_dbl,
+                                  c_dbl, c_dbl, c_uint, c_vp]),
     "lpm_variant_count": (c_int, [c_int]),
     "lpm_variant_default": (c_int, [c_int]),
     "lpm_variant_name": (ctypes.c_char_p, [c_int, c_int]),

@@ -32,11 +32,14 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-x", "hip",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fno-gpu-flush-denormals-to-zero",
+    # matrix-core operands and results in VGPRs: the VALU consumes every
+    # MFMA result, so accumulation registers would only add copies
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 # host-only sources: strict IEEE operation order for the mesh generator
 HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
 HIP_SOURCES = ["bve_sums.hip", "ensemble.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
-               "swe_rk4.hip"]
+               "swe_rk4.hip", "ic2d.hip"]
 EXAMPLES = ["bve_rotation", "sphere_swe_tc2"]
 
 

@@ -369,9 +369,11 @@ __global__ __launch_bounds__(kPackTile) void pack_mixed_kernel(
 //   GEMM1 (v_mfma_f32_16x16x4_f32, 16 sources x 16 targets x K = 4):
 //     r2 = |x'|^2 - 2 x'.y' + |y'|^2  with A_j = (y', 1),
 //     B_i = (-2 x', |x'|^2), C_ji = |y'_j|^2;
-//   VALU: w = 2 s / r2;
-//   GEMM2 (v_mfma_f32_4x4x1_16b_f32): sum_j w (y', 1) per target, i.e. the
-//     tile's sum of w y' and of w, carried into FP64 as S += S' + c W.
+//   VALU: 1 / r2;
+//   GEMM2 (v_mfma_f32_4x4x1_16b_f32): sum_j (1 / r2) 2 s_j (y', 1) per
+//     target (the weight 2 s is folded into the packed operand), i.e. the
+//     tile's sum of w y' and of w = 2 s / r2, carried into FP64 as
+//     S += S' + c W.
 // A wave (64 targets) uses the matrix path for a tile only when none of its
 // targets lies within kMfmaNear R of c: then r2 >= (kMfmaNear - 1)^2 R^2
 // while its FP32 terms are at most (kMfmaNear + 1)^2 R^2, which bounds the
@@ -384,8 +386,9 @@ __global__ __launch_bounds__(kPackTile) void pack_mixed_kernel(
 //   mf_y[j] = (y'_0, y'_1, y'_2, 1)       GEMM1 A and near-path positions
 //   mf_q[j] = |y'|^2                       GEMM1 C
 //   mf_s[j] = 2 s                          weights (0 in padding)
-//   mf_t[t 1024 + ((ch 4 + g) 4 + c) 4 + r] = component c of source
-//            256 t + 16 ch + 4 g + r (c = 3: 1), the GEMM2 A operand
+//   mf_t[t 1024 + ((ch 4 + g) 4 + c) 4 + r] = 2 s times component c of
+//            source 256 t + 16 ch + 4 g + r (c = 3: 2 s), the GEMM2 A
+//            operand
 //   mf_c[t] = (c, R^2)
 // Padding sources sit at y' = (8, 8, 8) with zero weight.
 constexpr double kMfmaNear = 3.0;
@@ -462,10 +465,10 @@ __global__ __launch_bounds__(kPackTile) void pack_mfma_kernel(
   // transposed GEMM2 operand: local index jj = 16 ch + 4 g + r
   const int jj = threadIdx.x, ch = jj >> 4, g = (jj >> 2) & 3, r = jj & 3;
   float* tt = mf_t + static_cast<int64_t>(blockIdx.x) * 1024 + (ch * 4 + g) * 16 + r;
-  tt[0] = y0;
-  tt[4] = y1;
-  tt[8] = y2;
-  tt[12] = 1.0f;
+  tt[0] = ws * y0;
+  tt[4] = ws * y1;
+  tt[8] = ws * y2;
+  tt[12] = ws;
 }
 
 // Block `blk` (256 targets, one per lane) of segment sg against nsrc
@@ -550,7 +553,6 @@ __device__ __forceinline__ void mfma_block(const MfmaSrc& src, int nsrc,
     for (int ch = 0; ch < kPackTile / 16; ++ch) {
       const float A1 = lyf[(16 * ch + col) * 4 + k];
       const float4 qv = reinterpret_cast<const float4*>(lq)[4 * ch + k];
-      const float4 sv = reinterpret_cast<const float4*>(ls)[4 * ch + k];
       const float4 tv = lt[(ch * 4 + k) * 4 + (lane & 3)];
       mf4 Cy = {qv.x, qv.y, qv.z, qv.w};
       if (EPS) Cy += mf4{eps2x2, eps2x2, eps2x2, eps2x2};
@@ -558,10 +560,10 @@ __device__ __forceinline__ void mfma_block(const MfmaSrc& src, int nsrc,
       for (int tb = 0; tb < 4; ++tb) {
         const mf4 r2 =
             __builtin_amdgcn_mfma_f32_16x16x4f32(A1, B[tb], Cy, 0, 0, 0);
-        const float w0 = sv.x * __builtin_amdgcn_rcpf(r2[0]);
-        const float w1 = sv.y * __builtin_amdgcn_rcpf(r2[1]);
-        const float w2 = sv.z * __builtin_amdgcn_rcpf(r2[2]);
-        const float w3 = sv.w * __builtin_amdgcn_rcpf(r2[3]);
+        const float w0 = __builtin_amdgcn_rcpf(r2[0]);
+        const float w1 = __builtin_amdgcn_rcpf(r2[1]);
+        const float w2 = __builtin_amdgcn_rcpf(r2[2]);
+        const float w3 = __builtin_amdgcn_rcpf(r2[3]);
         D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.x, w0, D2[tb], 0, 0, 0);
         D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.y, w1, D2[tb], 0, 0, 0);
         D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(tv.z, w2, D2[tb], 0, 0, 0);

@@ -138,28 +138,50 @@ __global__ __launch_bounds__(BLOCK) void ens_mixed_kernel(
   }
 }
 
+__global__ __launch_bounds__(256) void ens_mfma_kernel(MfmaSrc src, EnsArgs A) {
+  __shared__ float4 ly[kPackTile];
+  __shared__ float lq[kPackTile];
+  __shared__ float ls[kPackTile];
+  __shared__ float4 lt[kPackTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = ens_next(A, &slot);
+    if (it >= A.n_items) break;
+    const int4 w = A.items[it];
+    const EnsMember M = A.mem[w.x];
+    const SegArgs sg = ens_segment(A, M, w.y);
+    const int t0 = M.P / kPackTile;
+    MfmaSrc ms{src.y + M.P, src.q + M.P, src.s + M.P,
+               src.t + static_cast<int64_t>(t0) * 1024, src.c + t0};
+    mfma_block<false>(ms, M.cnt, sg, w.z, 0.0f, ly, lq, ls, lt);
+  }
+}
+
 // ------------------------------------------------------ kernel registry ---
 // Every variant has a one-problem twin of the same shape in bve_sums.hip.
 using EnsF64Fn = void (*)(const double4*, EnsArgs);
 using EnsMixedFn = void (*)(const float4*, const double4*, EnsArgs);
+using EnsMfmaFn = void (*)(MfmaSrc, EnsArgs);
 
 struct EnsVariant {
   EnsF64Fn f64;
   EnsMixedFn mixed;
   int block, tpl;
   const char* name;
+  EnsMfmaFn mfma;
   const void* fn() const {
     return f64 ? reinterpret_cast<const void*>(f64)
-               : reinterpret_cast<const void*>(mixed);
+               : mixed ? reinterpret_cast<const void*>(mixed)
+                       : reinterpret_cast<const void*>(mfma);
   }
 };
 
 #define LPM_ENS_F64(B, T, NR, PAIR)                  \
   {ens_f64_kernel<B, T, NR, PAIR>, nullptr, B, T,    \
-   "persistent_block" #B "_tpl" #T "_nr" #NR "_pair" #PAIR}
+   "persistent_block" #B "_tpl" #T "_nr" #NR "_pair" #PAIR, nullptr}
 #define LPM_ENS_MIXED(B, T, PAIR)                 \
   {nullptr, ens_mixed_kernel<B, T, PAIR>, B, T,   \
-   "persistent_block" #B "_tpl" #T "_pair" #PAIR}
+   "persistent_block" #B "_tpl" #T "_pair" #PAIR, nullptr}
 
 const EnsVariant kEnsF64[] = {
     LPM_ENS_F64(256, 2, 2, true),  // 0: fixed default
@@ -175,6 +197,8 @@ const EnsVariant kEnsMixed[] = {
     LPM_ENS_MIXED(256, 2, true),   // 3
     LPM_ENS_MIXED(256, 4, false),  // 4
     LPM_ENS_MIXED(128, 2, true),   // 5
+    {nullptr, nullptr, 256, 1, "persistent_mfma_block256_near3",
+     ens_mfma_kernel},             // 6: matrix-core far field
 };
 constexpr int kNumEnsF64 = sizeof(kEnsF64) / sizeof(kEnsF64[0]);
 constexpr int kNumEnsMixed = sizeof(kEnsMixed) / sizeof(kEnsMixed[0]);
@@ -213,6 +237,11 @@ void ensure_sources(lpm_ctx* ctx, const EnsembleDesc& d, int precision) {
   if (precision == LPM_MIXED) {
     ctx->packed32.ensure(sizeof(float4) * cap);
     ctx->centers.ensure(sizeof(double4) * (cap / kPackTile));
+    ctx->mf_y.ensure(sizeof(float4) * cap);
+    ctx->mf_q.ensure(sizeof(float) * cap);
+    ctx->mf_s.ensure(sizeof(float) * cap);
+    ctx->mf_t.ensure(sizeof(float) * 4 * cap);
+    ctx->mf_c.ensure(sizeof(double4) * (cap / kPackTile));
   }
 }
 
@@ -267,19 +296,33 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
   if (A.n_items == 0) return;
   const float4* p32 = nullptr;
   const double4* cen = nullptr;
+  MfmaSrc ms{};
   if (precision == LPM_MIXED) {
     const int ntile = static_cast<int>(d.total_padded_upper / kPackTile);
     p32 = ctx->packed32.as<float4>();
     cen = ctx->centers.as<double4>();
-    pack_mixed_kernel<<<ntile, kPackTile, 0, s>>>(
-        p64, L.padded_total, ctx->packed32.as<float4>(),
-        ctx->centers.as<double4>());
+    if (v.mfma) {
+      ms = MfmaSrc{ctx->mf_y.as<float4>(), ctx->mf_q.as<float>(),
+                   ctx->mf_s.as<float>(), ctx->mf_t.as<float>(),
+                   ctx->mf_c.as<double4>()};
+      // member padding is all-zero records, which the packer skips
+      pack_mfma_kernel<<<ntile, kPackTile, 0, s>>>(
+          p64, L.padded_total, ctx->mf_y.as<float4>(), ctx->mf_q.as<float>(),
+          ctx->mf_s.as<float>(), ctx->mf_t.as<float>(),
+          ctx->mf_c.as<double4>());
+    } else {
+      pack_mixed_kernel<<<ntile, kPackTile, 0, s>>>(
+          p64, L.padded_total, ctx->packed32.as<float4>(),
+          ctx->centers.as<double4>());
+    }
     LPM_HIP_CHECK(hipGetLastError());
   }
   ensemble_reset_queue(L, s);
   const int grid = ensemble_grid(ctx, v.fn(), v.block, A.n_items);
   TimedLaunch tl(ctx, family, s, L.pairs);
-  if (precision == LPM_MIXED)
+  if (v.mfma)
+    hipLaunchKernelGGL(v.mfma, dim3(grid), dim3(v.block), 0, s, ms, A);
+  else if (precision == LPM_MIXED)
     hipLaunchKernelGGL(v.mixed, dim3(grid), dim3(v.block), 0, s, p32, cen, A);
   else
     hipLaunchKernelGGL(v.f64, dim3(grid), dim3(v.block), 0, s,

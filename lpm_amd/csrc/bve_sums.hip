@@ -156,6 +156,20 @@ __global__ __launch_bounds__(BLOCK) void bve_vel_mixed_kernel(
                                            eps2x2, tile);
 }
 
+template <bool EPS>
+__global__ __launch_bounds__(256) void bve_vel_mfma_kernel(
+    MfmaSrc src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
+    float eps2x2) {
+  __shared__ float4 ly[kPackTile];
+  __shared__ float lq[kPackTile];
+  __shared__ float ls[kPackTile];
+  __shared__ float4 lt[kPackTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  mfma_block<EPS>(src, *count, sg, blk, eps2x2, ly, lq, ls, lt);
+}
+
 // ------------------------------------------------------ stream function ---
 // psi_i = sum_j s_j log(1 - x_i.y_j); FP64 log from the device math library.
 template <int BLOCK, int TILE>
@@ -220,6 +234,7 @@ using F64Kernel = void (*)(const double4*, const int32_t*, SegArgs, SegArgs,
                            double);
 using MixedKernel = void (*)(const float4*, const double4*, const int32_t*,
                              SegArgs, SegArgs, float);
+using MfmaKernel = void (*)(MfmaSrc, const int32_t*, SegArgs, SegArgs, float);
 
 struct F64Variant {
   F64Kernel fn, fn_eps;
@@ -227,7 +242,8 @@ struct F64Variant {
   const char* name;
 };
 struct MixedVariant {
-  MixedKernel fn, fn_eps;
+  MixedKernel fn, fn_eps;  // vector-ALU kernels (or null)
+  MfmaKernel mfn, mfn_eps; // matrix-core kernels (or null)
   int block, tpl;
   const char* name;
 };
@@ -238,7 +254,7 @@ struct MixedVariant {
    "block" #B "_tpl" #T "_nr" #NR "_pair" #PAIR}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
-   bve_vel_mixed_kernel<B, T, 256, true, PAIR>, B, T,                    \
+   bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
    "block" #B "_tpl" #T "_pair" #PAIR}
 
 const F64Variant kF64Variants[] = {
@@ -259,6 +275,8 @@ const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(128, 2, true),   // 4
     LPM_MIXED_VARIANT(256, 1, false),  // 5
     LPM_MIXED_VARIANT(256, 4, false),  // 6
+    {nullptr, nullptr, bve_vel_mfma_kernel<false>, bve_vel_mfma_kernel<true>,
+     256, 1, "mfma_block256_near3"},   // 7: matrix-core far field
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
@@ -401,6 +419,23 @@ PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
     LPM_HIP_CHECK(hipGetLastError());
     ps.data = p32;
     ps.centers = cen;
+    const int64_t tcap = static_cast<int64_t>(ntile) * kPackTile;
+    MfmaSrc m;
+    m.y = static_cast<float4*>(ctx->mf_y.ensure(sizeof(float4) * tcap));
+    m.q = static_cast<float*>(ctx->mf_q.ensure(sizeof(float) * tcap));
+    m.s = static_cast<float*>(ctx->mf_s.ensure(sizeof(float) * tcap));
+    m.t = static_cast<float*>(ctx->mf_t.ensure(sizeof(float) * 4 * tcap));
+    m.c = static_cast<double4*>(ctx->mf_c.ensure(sizeof(double4) * ntile));
+    pack_mfma_kernel<<<ntile, kPackTile, 0, s>>>(
+        p64, map.d_count, const_cast<float4*>(m.y), const_cast<float*>(m.q),
+        const_cast<float*>(m.s), const_cast<float*>(m.t),
+        const_cast<double4*>(m.c));
+    LPM_HIP_CHECK(hipGetLastError());
+    ps.mf_y = m.y;
+    ps.mf_q = m.q;
+    ps.mf_s = m.s;
+    ps.mf_t = m.t;
+    ps.mf_c = m.c;
   }
   return ps;
 }
@@ -435,9 +470,18 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
     TimedLaunch tl(ctx, KID_BVE_VEL_MIXED, s, pairs);
-    hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block), 0, s,
-                       static_cast<const float4*>(src.data), src.centers,
-                       src.count, a0, a1, static_cast<float>(2.0 * eps2));
+    if (v.mfn) {
+      LPM_ARG_CHECK(src.mf_y, "matrix-core layout not packed");
+      MfmaSrc m{src.mf_y, src.mf_q, src.mf_s, src.mf_t, src.mf_c};
+      hipLaunchKernelGGL(eps2 > 0 ? v.mfn_eps : v.mfn, dim3(grid),
+                         dim3(v.block), 0, s, m, src.count, a0, a1,
+                         static_cast<float>(2.0 * eps2));
+    } else {
+      hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block),
+                         0, s, static_cast<const float4*>(src.data),
+                         src.centers, src.count, a0, a1,
+                         static_cast<float>(2.0 * eps2));
+    }
     LPM_HIP_CHECK(hipGetLastError());
   } else {
     throw Error(LPM_ERR_ARG, "unknown precision");

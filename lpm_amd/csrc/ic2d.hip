@@ -1,0 +1,380 @@
+// 2-D incompressible flow on the sphere: the fused velocity + stream
+// function direct sum of Incompressible2DKernels<SphereGeometry>::kernel_vals
+// (src/lpm_incompressible2d_kernels.hpp:35-47) and the Heun RK2 stepper of
+// Incompressible2DRK2 (src/lpm_incompressible2d_rk2_impl.hpp:75-172).
+//
+// The sum shares the BVE prologue (mask scan, packed FP64 sources
+// {y, s = -zeta A / 4 pi}) and target segments.  Per pair:
+//   arg = 1 - x.y + eps^2,  S += (s / arg) y,  psi += s log(arg),
+// and u = x cross S at the end (x cross y / (-4 pi arg) Gamma = x cross
+// (s / arg) y).  The FP64 log dominates the pair cost.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <sstream>
+
+#include "bve_device.hpp"
+#include "internal.hpp"
+
+namespace lpm_amd {
+namespace {
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
+    const double4* __restrict__ src, const int32_t* __restrict__ count,
+    SegArgs s0, SegArgs s1, double eps2, int skip_self) {
+  __shared__ double4 tile[kPackTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  const int nsrc = *count;
+  double x[1][3];
+  int self[1];
+  bool live[1];
+  int64_t tgt[1];
+  if (!load_targets<BLOCK, 1>(sg, blk, x, self, live, tgt)) return;
+  if (!skip_self) self[0] = -1;
+  double S[1][3] = {{0.0, 0.0, 0.0}};
+  double psi = 0.0;
+  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kPackTile; t += BLOCK) {
+      const int j = j0 + t;
+      tile[t] = j < nsrc ? src[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    __syncthreads();
+    const bool diag = static_cast<unsigned>(self[0] - j0) <
+                      static_cast<unsigned>(kPackTile);
+    for (int jj = 0; jj < kPackTile; ++jj) {
+      const double4 y = tile[jj];
+      const double t = x[0][0] * y.x + x[0][1] * y.y + x[0][2] * y.z;
+      const double arg = 1.0 - t + eps2;
+      double w = y.w * rcp_nr<2>(arg);
+      double p = y.w * log(arg);
+      if (diag && j0 + jj == self[0]) {
+        w = 0.0;
+        p = 0.0;
+      }
+      S[0][0] = __builtin_fma(w, y.x, S[0][0]);
+      S[0][1] = __builtin_fma(w, y.y, S[0][1]);
+      S[0][2] = __builtin_fma(w, y.z, S[0][2]);
+      psi += p;
+    }
+  }
+  if (sg.out) store_cross<1>(sg, x, S, live, tgt);
+  if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
+}
+
+// Stage storage of one particle kind (SoA, leading dimension n): x1 (3n),
+// z1 (n), xw (3n), zw (n).
+struct Ic2dStage {
+  double* x1;
+  double* z1;
+  double* xw;
+  double* zw;
+  int64_t n;
+};
+
+Ic2dStage carve(DevBuf& buf, int64_t n) {
+  const int64_t nn = std::max<int64_t>(n, 1);
+  double* p = static_cast<double*>(buf.ensure(sizeof(double) * 8 * nn));
+  return Ic2dStage{p, p + 3 * nn, p + 4 * nn, p + 7 * nn, n};
+}
+
+struct Ic2dArgs {
+  double* x;  // state positions (strided)
+  int64_t xes, xcs;
+  double* zeta;
+  const double* u;  // velocity (strided)
+  int64_t ues, ucs;
+  const uint8_t* skip;
+  Ic2dStage st;
+  double dt, Omega;
+  int renorm;
+};
+
+__device__ __forceinline__ void project(double* v) {
+  const double r = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] *= r;
+  v[1] *= r;
+  v[2] *= r;
+}
+
+// Stage 1: x1 = dt u, z1 = -2 Omega u_z (a rate), xw = x + dt u,
+// zw = zeta + dt z1 (KokkosBlas::scal / update and
+// Incompressible2DTendencies of the reference's first stage).
+__global__ __launch_bounds__(256) void ic2d_stage1_kernel(Ic2dArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = A.st.n;
+  if (i >= n) return;
+  if (A.skip && A.skip[i]) return;
+  const double* ui = A.u + i * A.ues;
+  const double* xi = A.x + i * A.xes;
+  const double u[3] = {ui[0], ui[A.ucs], ui[2 * A.ucs]};
+  double w[3];
+  for (int k = 0; k < 3; ++k) {
+    A.st.x1[i + k * n] = A.dt * u[k];
+    w[k] = xi[k * A.xcs] + A.dt * u[k];
+  }
+  if (A.renorm) project(w);
+  for (int k = 0; k < 3; ++k) A.st.xw[i + k * n] = w[k];
+  const double z1 = -2.0 * A.Omega * u[2];
+  A.st.z1[i] = z1;
+  A.st.zw[i] = A.zeta[i] + A.dt * z1;
+}
+
+// Stage 2 and the Heun update: z2 = -2 Omega u2_z;
+// zeta += dt/2 z1 + dt/2 z2, x += x1/2 + (dt u2)/2.
+__global__ __launch_bounds__(256) void ic2d_update_kernel(Ic2dArgs A) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = A.st.n;
+  if (i >= n) return;
+  if (A.skip && A.skip[i]) return;
+  const double* ui = A.u + i * A.ues;
+  double* xi = A.x + i * A.xes;
+  const double u[3] = {ui[0], ui[A.ucs], ui[2 * A.ucs]};
+  const double z2 = -2.0 * A.Omega * u[2];
+  A.zeta[i] += 0.5 * A.dt * A.st.z1[i] + 0.5 * A.dt * z2;
+  double v[3];
+  for (int k = 0; k < 3; ++k)
+    v[k] = xi[k * A.xcs] + (0.5 * A.st.x1[i + k * n] + 0.5 * (A.dt * u[k]));
+  if (A.renorm) project(v);
+  for (int k = 0; k < 3; ++k) xi[k * A.xcs] = v[k];
+}
+
+inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
+
+void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
+                 int nseg, double eps, bool collocated_skip, hipStream_t s) {
+  TargetSeg empty;
+  const TargetSeg& t0 = segs[0];
+  const TargetSeg& t1 = nseg > 1 ? segs[1] : empty;
+  if (t0.n + t1.n == 0) return;
+  constexpr int kBlock = 256;
+  auto to_args = [](const TargetSeg& t) {
+    SegArgs a{};
+    a.x = t.x.base;
+    a.es = t.x.es;
+    a.cs = t.x.cs;
+    a.n = t.n;
+    a.index = t.index;
+    a.d_n = t.d_n;
+    a.self_rank = t.self_rank;
+    a.self_identity = t.self_identity ? 1 : 0;
+    a.nblocks = static_cast<int>((t.n + kBlock - 1) / kBlock);
+    a.skip = t.skip;
+    a.out = t.out.base;
+    a.oes = t.out.es;
+    a.ocs = t.out.cs;
+    a.out_scalar = t.out_scalar;
+    return a;
+  };
+  SegArgs a0 = to_args(t0), a1 = to_args(t1);
+  const double pairs = static_cast<double>(t0.n + t1.n) * src.capacity;
+  TimedLaunch tl(ctx, KID_IC2D, s, pairs);
+  hipLaunchKernelGGL((ic2d_f64_kernel<kBlock>), dim3(a0.nblocks + a1.nblocks),
+                     dim3(kBlock), 0, s,
+                     static_cast<const double4*>(src.data), src.count, a0, a1,
+                     eps * eps, collocated_skip ? 1 : 0);
+  LPM_HIP_CHECK(hipGetLastError());
+}
+
+// Sums of the fields at positions (vx, fx) with face vorticity fzeta.
+void ic2d_eval(lpm_ctx* ctx, const lpm_ic2d_fields& f, CrdIn vx, CrdIn fx,
+               const double* fzeta, const RankMap& map, double eps,
+               unsigned flags, hipStream_t s) {
+  PackedSources ps = pack_sources(ctx, fx, fzeta, f.face_area, map, LPM_FP64, s);
+  TargetSeg segs[2];
+  segs[0].x = vx;
+  segs[0].n = f.n_verts;
+  segs[0].out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
+  segs[0].out_scalar = f.vert_psi;
+  segs[1] = face_targets(fx, map, f.face_mask,
+                         (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
+                         CrdOut{f.face_u, f.face_u_es, f.face_u_cs},
+                         f.face_psi);
+  // the self-exclusion of segment 1 applies only when eps == 0
+  launch_ic2d(ctx, ps, segs, 2, eps, eps == 0.0, s);
+}
+
+void validate(const lpm_ic2d_fields* f) {
+  LPM_ARG_CHECK(f, "null fields");
+  LPM_ARG_CHECK(f->n_verts >= 0 && f->n_faces > 0, "bad sizes");
+  if (f->n_verts > 0)
+    LPM_ARG_CHECK(f->vert_x && f->vert_zeta && f->vert_u && f->vert_x_es > 0 &&
+                      f->vert_x_cs > 0 && f->vert_u_es > 0 && f->vert_u_cs > 0,
+                  "vertex fields incomplete");
+  LPM_ARG_CHECK(f->face_x && f->face_zeta && f->face_u && f->face_area &&
+                    f->face_x_es > 0 && f->face_x_cs > 0 && f->face_u_es > 0 &&
+                    f->face_u_cs > 0,
+                "face fields incomplete");
+}
+
+void issue_rk2(lpm_ctx* ctx, const lpm_ic2d_fields& f, double dt, double Omega,
+               double eps, unsigned flags, Ic2dStage& V, Ic2dStage& F,
+               hipStream_t s) {
+  const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
+  const uint8_t* fskip =
+      (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
+  const int renorm = (flags & LPM_RK4_RENORMALIZE) ? 1 : 0;
+  auto args = [&](bool face) {
+    Ic2dArgs a{};
+    a.x = face ? f.face_x : f.vert_x;
+    a.xes = face ? f.face_x_es : f.vert_x_es;
+    a.xcs = face ? f.face_x_cs : f.vert_x_cs;
+    a.zeta = face ? f.face_zeta : f.vert_zeta;
+    a.u = face ? f.face_u : f.vert_u;
+    a.ues = face ? f.face_u_es : f.vert_u_es;
+    a.ucs = face ? f.face_u_cs : f.vert_u_cs;
+    a.skip = face ? fskip : nullptr;
+    a.st = face ? F : V;
+    a.dt = dt;
+    a.Omega = Omega;
+    a.renorm = renorm;
+    return a;
+  };
+  if (f.n_verts > 0) {
+    ic2d_stage1_kernel<<<nblk(f.n_verts), 256, 0, s>>>(args(false));
+    LPM_HIP_CHECK(hipGetLastError());
+  }
+  ic2d_stage1_kernel<<<nblk(f.n_faces), 256, 0, s>>>(args(true));
+  LPM_HIP_CHECK(hipGetLastError());
+  ic2d_eval(ctx, f, CrdIn{V.xw, 1, V.n}, CrdIn{F.xw, 1, F.n}, F.zw, map, eps,
+            flags, s);
+  if (f.n_verts > 0) {
+    ic2d_update_kernel<<<nblk(f.n_verts), 256, 0, s>>>(args(false));
+    LPM_HIP_CHECK(hipGetLastError());
+  }
+  ic2d_update_kernel<<<nblk(f.n_faces), 256, 0, s>>>(args(true));
+  LPM_HIP_CHECK(hipGetLastError());
+  ic2d_eval(ctx, f, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
+            CrdIn{f.face_x, f.face_x_es, f.face_x_cs}, f.face_zeta, map, eps,
+            flags, s);
+}
+
+std::string ic2d_key(const lpm_ic2d_fields& f, double dt, double Omega,
+                     double eps, unsigned flags, hipStream_t s) {
+  std::ostringstream k;
+  k << "ic2d|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
+    << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_u << '|'
+    << f.vert_u_es << '|' << f.vert_u_cs << '|' << f.vert_psi << '|'
+    << f.n_faces << '|' << f.face_x << '|' << f.face_x_es << '|'
+    << f.face_x_cs << '|' << f.face_zeta << '|' << f.face_u << '|'
+    << f.face_u_es << '|' << f.face_u_cs << '|' << f.face_psi << '|'
+    << f.face_area << '|' << static_cast<const void*>(f.face_mask) << '|';
+  k.precision(17);
+  k << dt << '|' << Omega << '|' << eps << '|' << flags << '|' << s;
+  return k.str();
+}
+
+template <typename F>
+int guarded(lpm_ctx* ctx, F&& fn) {
+  try {
+    fn();
+    return LPM_OK;
+  } catch (const Error& e) {
+    set_error(ctx, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(ctx, e.what());
+    return LPM_ERR_INTERNAL;
+  }
+}
+
+}  // namespace
+}  // namespace lpm_amd
+
+using namespace lpm_amd;
+
+extern "C" {
+
+int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                  int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                  int64_t src_es, int64_t src_cs, const double* src_zeta,
+                  const double* src_area, const uint8_t* src_mask,
+                  int64_t n_src, int collocated, double eps, unsigned flags,
+                  double* out_u, int64_t out_es, int64_t out_cs,
+                  double* out_psi, void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    LPM_ARG_CHECK(n_tgt >= 0 && n_src >= 0, "negative size");
+    LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
+    if (n_tgt > 0) {
+      LPM_ARG_CHECK(tgt_x && tgt_es > 0 && tgt_cs > 0, "bad tgt_x");
+      LPM_ARG_CHECK(!out_u || (out_es > 0 && out_cs > 0), "bad out_u strides");
+    }
+    if (n_src > 0) {
+      LPM_ARG_CHECK(src_x && src_es > 0 && src_cs > 0, "bad src_x");
+      LPM_ARG_CHECK(src_zeta && src_area, "src_zeta/src_area are NULL");
+    }
+    if (collocated)
+      LPM_ARG_CHECK(n_tgt == n_src,
+                    "collocated: targets must be the sources (n_tgt == n_src)");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, src_mask, n_src, s);
+    PackedSources ps = pack_sources(ctx, CrdIn{src_x, src_es, src_cs}, src_zeta,
+                                    src_area, map, LPM_FP64, s);
+    TargetSeg t;
+    if (collocated) {
+      t = face_targets(CrdIn{tgt_x, tgt_es, tgt_cs}, map, src_mask,
+                       (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
+                       CrdOut{out_u, out_es, out_cs}, out_psi);
+    } else {
+      t.x = CrdIn{tgt_x, tgt_es, tgt_cs};
+      t.n = n_tgt;
+      t.out = CrdOut{out_u, out_es, out_cs};
+      t.out_scalar = out_psi;
+    }
+    launch_ic2d(ctx, ps, &t, 1, eps, collocated && eps == 0.0, s);
+    if ((flags & LPM_FLAG_CHECK_FINITE) && out_u)
+      check_finite(ctx, out_u, n_tgt, out_es, out_cs, 3, s);
+  });
+}
+
+int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
+                              double eps, unsigned flags, void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+    ic2d_eval(ctx, *f, CrdIn{f->vert_x, f->vert_x_es, f->vert_x_cs},
+              CrdIn{f->face_x, f->face_x_es, f->face_x_cs}, f->face_zeta, map,
+              eps, flags, s);
+  });
+}
+
+int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
+                      double Omega, double eps, unsigned flags, void* stream) {
+  return guarded(ctx, [&] {
+    LPM_ARG_CHECK(ctx, "null context");
+    validate(f);
+    LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
+                  "dt and Omega must be finite");
+    LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    Ic2dStage V = carve(ctx->ic2d_v, f->n_verts);
+    Ic2dStage F = carve(ctx->ic2d_f, f->n_faces);
+    run_maybe_graph(
+        ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
+        [&]() {
+          const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
+          pack_sources(ctx, CrdIn{f->face_x, f->face_x_es, f->face_x_cs},
+                       f->face_zeta, f->face_area, map, LPM_FP64, s);
+          return ic2d_key(*f, dt, Omega, eps, flags, s);
+        },
+        [&]() { issue_rk2(ctx, *f, dt, Omega, eps, flags, V, F, s); });
+    if (flags & LPM_FLAG_CHECK_FINITE) {
+      check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
+      check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);
+    }
+  });
+}
+
+}  // extern "C"

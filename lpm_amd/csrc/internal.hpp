@@ -103,6 +103,12 @@ struct PackedSources {
   const double4* centers = nullptr; // mixed: tile centres (x, y, z, -)
   const int32_t* count = nullptr;  // device: number of packed sources
   int64_t capacity = 0;            // host upper bound (n stored faces)
+  // mixed: matrix-core layout (see bve_device.hpp)
+  const float4* mf_y = nullptr;
+  const float* mf_q = nullptr;
+  const float* mf_s = nullptr;
+  const float* mf_t = nullptr;
+  const double4* mf_c = nullptr;
 };
 
 // Kernel timing records (HIP events around direct-sum launches).
@@ -114,7 +120,8 @@ enum KernelId : int {
   KID_SWE_MIXED = 4,
   KID_BVE_ENS_F64 = 5,
   KID_BVE_ENS_MIXED = 6,
-  KID_COUNT = 7
+  KID_IC2D = 7,
+  KID_COUNT = 8
 };
 
 struct TimingRec {
@@ -144,6 +151,7 @@ struct lpm_ctx {
   lpm_amd::DevBuf packed64;     // double4 per source
   lpm_amd::DevBuf packed32;     // float4 per source (mixed)
   lpm_amd::DevBuf centers;      // double4 per tile (mixed)
+  lpm_amd::DevBuf mf_y, mf_q, mf_s, mf_t, mf_c;  // matrix-core layout (mixed)
   lpm_amd::DevBuf flag;         // int32 non-finite flag
   lpm_amd::DevBuf count_ident;  // int32 count when no mask
 
@@ -160,6 +168,7 @@ struct lpm_ctx {
   // RK4 stage buffers
   lpm_amd::DevBuf rk_v, rk_f;   // BVE vertex / face stage storage
   lpm_amd::DevBuf swe_v, swe_f; // SWE vertex / face stage storage
+  lpm_amd::DevBuf ic2d_v, ic2d_f; // Incompressible2D RK2 stage storage
 
   // timing
   bool timing = false;
@@ -172,7 +181,7 @@ struct lpm_ctx {
 
   // autotuned kernel variants, keyed by (arch, kernel, size bucket)
   std::map<std::string, int> tuned;
-  int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1, -1};
+  int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 namespace lpm_amd {

@@ -147,6 +147,35 @@ class Context:
             stream if stream is not None else current_stream()))
         return best.value, ms.value, list(all_ms)
 
+    # --------------------------------------------------- Incompressible2D
+    def ic2d_sums(self, tgt_x, src_x, src_zeta, src_area, src_mask=None,
+                  collocated=False, eps=0.0, flags=0, out_u=None, out_psi=None,
+                  stream=None):
+        """Fused velocity (3, n) and stream function (n) at the targets."""
+        tp, tes, tcs, nt = crd_args(tgt_x)
+        sp, ses, scs, ns = crd_args(src_x)
+        if out_u is None:
+            out_u = torch.zeros((3, nt), dtype=torch.float64, device=tgt_x.device)
+        if out_psi is None:
+            out_psi = torch.zeros(nt, dtype=torch.float64, device=tgt_x.device)
+        op, oes, ocs, _ = crd_args(out_u)
+        self.check(self.L.lpm_ic2d_sums(
+            self.h, tp, tes, tcs, nt, sp, ses, scs, _ptr(src_zeta),
+            _ptr(src_area), _ptr(src_mask), ns, int(collocated), float(eps),
+            flags, op, oes, ocs, _ptr(out_psi),
+            stream if stream is not None else current_stream()))
+        return out_u, out_psi
+
+    def ic2d_init_direct_sums(self, fields, eps, flags=0, stream=None):
+        self.check(self.L.lpm_ic2d_init_direct_sums(
+            self.h, ctypes.byref(fields), float(eps), flags,
+            stream if stream is not None else current_stream()))
+
+    def ic2d_rk2_step(self, fields, dt, Omega, eps, flags=0, stream=None):
+        self.check(self.L.lpm_ic2d_rk2_step(
+            self.h, ctypes.byref(fields), float(dt), float(Omega), float(eps),
+            flags, stream if stream is not None else current_stream()))
+
     # ----------------------------------------------------------- SWE sums
     def swe_sums(self, tgt_x, tgt_surf, src_x, src_zeta, src_sigma, src_area,
                  src_surf, src_mask=None, collocated=False, eps=0.0,

@@ -414,6 +414,120 @@ void ora_swe_sums(const double* tx, const double* ts, int64_t nt,
   }
 }
 
+/* Incompressible2DKernels<SphereGeometry>::kernel_vals summed by
+ * Incompressible2DReducer (src/lpm_incompressible2d_kernels.hpp:35-47,
+ * 96-136): with arg = 1 - x.y + eps^2 and Gamma = zeta A,
+ *   u   += [(x cross y) / (-4 pi arg)] Gamma,
+ *   psi += [-log(arg) / (4 pi)] Gamma,
+ * masked sources skipped, in source order.  Incompressible2DPassiveSums
+ * (collocated = 0, :143-190) never skips j == i; Incompressible2DActiveSums
+ * (collocated = 1, :197-247) skips j == i only when eps == 0 (:242).
+ * Outputs u (nt, 3) and psi (nt), either may be NULL.
+ * Pinned by: at eps = 0 the velocity equals ora_bve_velocity and psi equals
+ * ora_bve_stream_fn (restated independently), the RH 5-4 eigenrelation for
+ * psi, and a brute-force evaluation of the pair formula with eps > 0. */
+void ora_ic2d_sums(const double* tx, int64_t nt, const double* sx,
+                   const double* szeta, const double* sarea,
+                   const uint8_t* smask, int64_t ns, int collocated,
+                   double eps, double* u, double* psi, int nthreads) {
+  set_threads(nthreads);
+  const int skip_self = collocated && eps == 0.0;
+  const double e2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < nt; ++i) {
+    double s[4] = {0, 0, 0, 0};
+    const double* x = tx + 3 * i;
+    for (int64_t j = 0; j < ns; ++j) {
+      if (skip_self && i == j) continue;
+      if (smask && smask[j]) continue;
+      const double* y = sx + 3 * j;
+      const double gamma = szeta[j] * sarea[j];
+      const double arg = 1 - dot3(x, y) + e2;
+      double r[4];
+      r[0] = x[1] * y[2] - x[2] * y[1];
+      r[1] = x[2] * y[0] - x[0] * y[2];
+      r[2] = x[0] * y[1] - x[1] * y[0];
+      for (int k = 0; k < 3; ++k) r[k] /= (-4 * ORA_PI * arg);
+      r[3] = -log(arg) / (4 * ORA_PI);
+      for (int k = 0; k < 4; ++k) s[k] += r[k] * gamma;
+    }
+    if (u)
+      for (int k = 0; k < 3; ++k) u[3 * i + k] = s[k];
+    if (psi) psi[i] = s[3];
+  }
+}
+
+/* One Incompressible2DRK2::advance_timestep_impl on the sphere
+ * (src/lpm_incompressible2d_rk2_impl.hpp:75-172, Heun's method) with
+ * Incompressible2DTendencies (src/lpm_incompressible2d_kernels.hpp:256-277,
+ * dzeta/dt = -2 Omega u_z, CoriolisSphere):
+ *   x1 = dt u, z1 = -2 Omega u_z; xw = x + dt u, zw = z + dt z1;
+ *   sums at (xw, zw) -> u2; x2 = dt u2, z2 = -2 Omega u2_z;
+ *   z += dt/2 (z1 + z2), x += (x1 + x2)/2; sums at the new state.
+ * Every stored face is advanced (the reference's ranges are all faces).
+ * On entry vu/fu hold the velocity at the current state; on exit the
+ * velocity and stream function (vpsi/fpsi) at the new state.
+ * Pinned by: exact solid-body rotation to second order, RK2
+ * self-convergence order 2. */
+void ora_ic2d_rk2_step(double* vx, double* vzeta, double* vu, double* vpsi,
+                       int64_t nv, double* fx, double* fzeta, double* fu,
+                       double* fpsi, const double* farea, const uint8_t* fmask,
+                       int64_t nf, double dt, double Omega, double eps,
+                       int nthreads) {
+  double* vx1 = calloc(3 * (nv > 0 ? nv : 1), sizeof(double));
+  double* vz1 = calloc((nv > 0 ? nv : 1), sizeof(double));
+  double* vxw = calloc(3 * (nv > 0 ? nv : 1), sizeof(double));
+  double* vzw = calloc((nv > 0 ? nv : 1), sizeof(double));
+  double* fx1 = calloc(3 * nf, sizeof(double));
+  double* fz1 = calloc(nf, sizeof(double));
+  double* fxw = calloc(3 * nf, sizeof(double));
+  double* fzw = calloc(nf, sizeof(double));
+  for (int64_t i = 0; i < nv; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      vx1[3 * i + k] = dt * vu[3 * i + k];
+      vxw[3 * i + k] = vx[3 * i + k] + dt * vu[3 * i + k];
+    }
+    vz1[i] = -2.0 * Omega * vu[3 * i + 2];
+    vzw[i] = vzeta[i] + dt * vz1[i];
+  }
+  for (int64_t i = 0; i < nf; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      fx1[3 * i + k] = dt * fu[3 * i + k];
+      fxw[3 * i + k] = fx[3 * i + k] + dt * fu[3 * i + k];
+    }
+    fz1[i] = -2.0 * Omega * fu[3 * i + 2];
+    fzw[i] = fzeta[i] + dt * fz1[i];
+  }
+  ora_ic2d_sums(vxw, nv, fxw, fzw, farea, fmask, nf, 0, eps, vu, vpsi,
+                nthreads);
+  ora_ic2d_sums(fxw, nf, fxw, fzw, farea, fmask, nf, 1, eps, fu, fpsi,
+                nthreads);
+  for (int64_t i = 0; i < nv; ++i) {
+    const double z2 = -2.0 * Omega * vu[3 * i + 2];
+    vzeta[i] += 0.5 * dt * vz1[i] + 0.5 * dt * z2;
+    for (int k = 0; k < 3; ++k)
+      vx[3 * i + k] += 0.5 * vx1[3 * i + k] + 0.5 * (dt * vu[3 * i + k]);
+  }
+  for (int64_t i = 0; i < nf; ++i) {
+    const double z2 = -2.0 * Omega * fu[3 * i + 2];
+    fzeta[i] += 0.5 * dt * fz1[i] + 0.5 * dt * z2;
+    for (int k = 0; k < 3; ++k)
+      fx[3 * i + k] += 0.5 * fx1[3 * i + k] + 0.5 * (dt * fu[3 * i + k]);
+  }
+  ora_ic2d_sums(vx, nv, fx, fzeta, farea, fmask, nf, 0, eps, vu, vpsi,
+                nthreads);
+  ora_ic2d_sums(fx, nf, fx, fzeta, farea, fmask, nf, 1, eps, fu, fpsi,
+                nthreads);
+  free(vx1);
+  free(vz1);
+  free(vxw);
+  free(vzw);
+  free(fx1);
+  free(fz1);
+  free(fxw);
+  free(fzw);
+}
+
 /* CoriolisSphere, src/lpm_coriolis.hpp:154-195. */
 static inline double cor_f(const double* x, double Om) { return 2 * Om * x[2]; }
 static inline double cor_dfdt(const double* u, double Om) { return 2 * Om * u[2]; }

@@ -148,6 +148,39 @@ def bve_stream_fn(tx, sx, szeta, sarea, smask=None, collocated=False,
     return psi
 
 
+def ic2d_sums(tx, sx, szeta, sarea, smask=None, collocated=False, eps=0.0,
+              nthreads=0):
+    """Incompressible2D sphere sums: (velocity (n, 3), stream function (n))."""
+    tx, sx = _c3(tx), _c3(sx)
+    szeta, sarea = _c1(szeta), _c1(sarea)
+    smask = _c1(smask, np.uint8)
+    u = np.zeros((len(tx), 3))
+    psi = np.zeros(len(tx))
+    lib().ora_ic2d_sums(_d(tx), _i64(len(tx)), _d(sx), _d(szeta), _d(sarea),
+                        _u8(smask), _i64(len(sx)), ctypes.c_int(int(collocated)),
+                        ctypes.c_double(eps), _d(u), _d(psi),
+                        ctypes.c_int(nthreads))
+    return u, psi
+
+
+def ic2d_rk2_step(state: dict, dt, Omega, eps, nthreads=0):
+    """One Incompressible2DRK2 step of ``state`` (modified in place).
+
+    Keys: vx (nv,3), vzeta, vu (nv,3), vpsi, fx (nf,3), fzeta, fu (nf,3),
+    fpsi, farea, fmask.  vu/fu must hold the velocity at the current state.
+    """
+    s = state
+    for k in ("vx", "vu", "fx", "fu"):
+        assert s[k].flags["C_CONTIGUOUS"] and s[k].dtype == np.float64
+    lib().ora_ic2d_rk2_step(
+        _d(s["vx"]), _d(s["vzeta"]), _d(s["vu"]), _d(s["vpsi"]),
+        _i64(len(s["vx"])), _d(s["fx"]), _d(s["fzeta"]), _d(s["fu"]),
+        _d(s["fpsi"]), _d(s["farea"]), _u8(s.get("fmask")), _i64(len(s["fx"])),
+        ctypes.c_double(dt), ctypes.c_double(Omega), ctypes.c_double(eps),
+        ctypes.c_int(nthreads))
+    return state
+
+
 def bve_velocity_xp(tx, sx, szeta, sarea, smask=None, collocated=False,
                     nthreads=0):
     tx, sx = _c3(tx), _c3(sx)

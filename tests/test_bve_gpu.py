@@ -319,6 +319,33 @@ def test_every_variant_matches_oracle(ctx, ora, kid, prec):
         ctx.set_variant(kid, -1)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_matrix_core_variant_on_a_larger_mesh(ctx, seed):
+    # level 5: most source tiles are far from every wave, so most pairs take
+    # the matrix-core path; the near tiles take the difference form
+    m, z = _inputs(seed, 5)
+    vx, fx = dev(m.vert_x), dev(m.face_x)
+    args = (dev(z), dev(m.face_area), dev(m.face_mask, torch.uint8))
+    kid = _abi.LPM_KERNEL_BVE_VEL_MIXED
+    names = ctx.variants(kid)
+    mv = [i for i, n in enumerate(names) if n.startswith("mfma")]
+    assert mv, names
+    u64v = ctx.bve_velocity(vx, fx, *args).cpu().numpy()
+    u64f = ctx.bve_velocity(fx, fx, *args, collocated=True).cpu().numpy()
+    leaf = m.face_mask == 0
+    try:
+        for v in mv:
+            ctx.set_variant(kid, v)
+            uv = ctx.bve_velocity(vx, fx, *args, precision="mixed").cpu().numpy()
+            uf = ctx.bve_velocity(fx, fx, *args, collocated=True,
+                                  precision="mixed").cpu().numpy()
+            assert rel_l2(uv, u64v) < TOL_MIXED, names[v]
+            assert rel_l2(uf[:, leaf], u64f[:, leaf]) < TOL_MIXED, names[v]
+            assert rel_linf(uv, u64v) < 10 * TOL_MIXED, names[v]
+    finally:
+        ctx.set_variant(kid, -1)
+
+
 def test_regularised_kernel_matches_pair_formula(ctx, ora):
     # eps > 0: the pair term is kzeta_sphere with eps (the BVE kernel with
     # 1 - x.y + eps^2 in the denominator)

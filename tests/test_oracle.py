@@ -371,3 +371,101 @@ def test_err_norms(ora):
     np.testing.assert_allclose(l2, np.sqrt((0.25 * 2 + 1) / (1 + 12.5 + 4)),
                                rtol=1e-14)
     np.testing.assert_allclose(li, 1.0 / 2.5, rtol=1e-14)
+
+
+# ------------------------------------------------------- Incompressible2D
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ic2d_sums_equal_bve_sums_at_eps0(ora, seed):
+    # at eps = 0 the fused sum is the BVE velocity and stream function, which
+    # the oracle restates independently (biot_savart, greens_fn)
+    m = mesh(seed, 3)
+    from lpm_amd import galleries as G
+    z = G.rh54_vorticity(m.face_x, 0.1, -0.1)
+    vx, fx = aos(m.vert_x), aos(m.face_x)
+    u, psi = ora.ic2d_sums(vx, fx, z, m.face_area, m.face_mask)
+    assert rel_l2(u, ora.bve_velocity(vx, fx, z, m.face_area, m.face_mask)) < 1e-14
+    assert rel_l2(psi, ora.bve_stream_fn(vx, fx, z, m.face_area, m.face_mask)) < 1e-14
+    lf = m.face_mask == 0
+    u, psi = ora.ic2d_sums(fx, fx, z, m.face_area, m.face_mask, collocated=True)
+    ub = ora.bve_velocity(fx, fx, z, m.face_area, m.face_mask, collocated=True)
+    pb = ora.bve_stream_fn(fx, fx, z, m.face_area, m.face_mask, collocated=True)
+    assert rel_l2(u[lf], ub[lf]) < 1e-14
+    assert rel_l2(psi[lf], pb[lf]) < 1e-14
+
+
+def test_ic2d_velocity_with_eps_is_kzeta_sum(ora):
+    # the smoothed velocity term is kzeta_sphere, pinned by the reference's
+    # golden vectors
+    m = mesh(0, 1)
+    rng = np.random.default_rng(5)
+    z = rng.normal(size=m.n_faces)
+    vx, fx = aos(m.vert_x), aos(m.face_x)
+    eps = 0.07
+    u, _ = ora.ic2d_sums(vx, fx, z, m.face_area, m.face_mask, eps=eps)
+    ref = np.zeros_like(vx)
+    for i in range(len(vx)):
+        for j in range(len(fx)):
+            if not m.face_mask[j]:
+                ref[i] += ora.kzeta_sphere(vx[i], fx[j], z[j], m.face_area[j], eps)
+    assert rel_l2(u, ref) < 1e-14
+
+
+def test_ic2d_single_source_closed_forms(ora):
+    x = np.array([[0.6, 0.0, 0.8]])
+    y = np.array([[0.0, 0.6, 0.8]])
+    zeta, area, eps = np.array([1.7]), np.array([0.3]), 0.2
+    gamma = 1.7 * 0.3
+    _, psi = ora.ic2d_sums(x, y, zeta, area, eps=eps)
+    arg = 1 - 0.64 + eps * eps
+    assert psi[0] == pytest.approx(-np.log(arg) * gamma / (4 * np.pi), rel=1e-15)
+    # collocated: the self term is kept for eps > 0 (-log(eps^2) Gamma / 4 pi;
+    # x cross x = 0) and skipped for eps = 0
+    u, psi = ora.ic2d_sums(x, x, zeta, area, collocated=True, eps=eps)
+    assert psi[0] == pytest.approx(-np.log(eps * eps) * gamma / (4 * np.pi),
+                                   rel=1e-12)
+    assert np.abs(u).max() < 1e-16
+    u, psi = ora.ic2d_sums(x, x, zeta, area, collocated=True, eps=0.0)
+    assert psi[0] == 0.0 and np.abs(u).max() == 0.0
+
+
+def _ic2d_state(ora, m, zeta_fn, eps):
+    vx, fx = aos(m.vert_x).copy(), aos(m.face_x).copy()
+    s = {"vx": vx, "fx": fx, "vzeta": zeta_fn(m.vert_x).copy(),
+         "fzeta": zeta_fn(m.face_x).copy(), "farea": m.face_area.copy(),
+         "fmask": m.face_mask.copy()}
+    s["vu"], s["vpsi"] = ora.ic2d_sums(vx, fx, s["fzeta"], s["farea"],
+                                       s["fmask"], eps=eps)
+    s["fu"], s["fpsi"] = ora.ic2d_sums(fx, fx, s["fzeta"], s["farea"],
+                                       s["fmask"], collocated=True, eps=eps)
+    return s
+
+
+def test_ic2d_rk2_conserves_absolute_vorticity(ora):
+    # Heun's method advances zeta and z with the same two velocities, so
+    # zeta + 2 Omega z is invariant to rounding on every particle
+    m = mesh(1, 2)
+    Omega = 2 * np.pi
+    tilt = np.array([0.6, 0.0, 0.8])
+    s = _ic2d_state(ora, m, lambda x: 2 * 1.5 * (tilt @ np.asarray(x)), 0.05)
+    inv_v = s["vzeta"] + 2 * Omega * s["vx"][:, 2]
+    inv_f = s["fzeta"] + 2 * Omega * s["fx"][:, 2]
+    for _ in range(3):
+        ora.ic2d_rk2_step(s, 0.01, Omega, 0.05)
+    np.testing.assert_allclose(s["vzeta"] + 2 * Omega * s["vx"][:, 2], inv_v,
+                               rtol=0, atol=1e-13)
+    np.testing.assert_allclose(s["fzeta"] + 2 * Omega * s["fx"][:, 2], inv_f,
+                               rtol=0, atol=1e-13)
+
+
+def test_ic2d_rk2_second_order(ora):
+    # self-convergence of positions at T = 0.08 under solid-body rotation
+    m = mesh(1, 2)
+    zfn = lambda x: 2 * 2 * np.pi * np.asarray(x)[2]  # noqa: E731
+    ends = []
+    for n in (16, 32, 64):
+        s = _ic2d_state(ora, m, zfn, 0.0)
+        for _ in range(n):
+            ora.ic2d_rk2_step(s, 0.08 / n, 0.0, 0.0)
+        ends.append(s["vx"].copy())
+    r = np.abs(ends[0] - ends[1]).max() / np.abs(ends[1] - ends[2]).max()
+    assert 3.5 < r < 4.5
