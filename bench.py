@@ -395,6 +395,71 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
     return out
 
 
+def ic2d_run(ctx, steps=10, warmup=1, dt=0.005, level=6, oracle_stride=64):
+    """Incompressible2D on the sphere: fused FP64 velocity + stream function
+    sums, Heun RK2 (2 evaluations per step), RH 5-4, icosahedral level 6."""
+    import numpy as np
+    import torch
+    from lpm_amd import _abi
+    from lpm_amd import galleries as G
+    from lpm_amd.incompressible2d import Incompressible2D, Incompressible2DRK2
+    from lpm_amd.mesh import SphereMesh
+
+    mesh = SphereMesh.build("icos", level)
+    s = Incompressible2D("icos", level, ctx=ctx, mesh=mesh)
+    s.init_vorticity(lambda x: G.rh54_vorticity(x, 0.1, -0.1))
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    s.init_direct_sums(skip)
+    torch.cuda.synchronize()
+    u0 = s.velocity_passive.cpu().numpy()
+    p0 = s.stream_fn_passive.cpu().numpy()
+    stream = torch.cuda.Stream()
+    r = Incompressible2DRK2(dt, s, leaf_only=True, renormalize=True,
+                            use_graph=True, stream=stream.cuda_stream)
+    for _ in range(warmup):
+        s.advance_timestep(r)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.advance_timestep(r)
+    stream.synchronize()
+    el = time.perf_counter() - t0
+    pairs = mesh.useful_pairs()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sl = slice(0, mesh.n_verts, oracle_stride)
+    z = G.rh54_vorticity(mesh.face_x, 0.1, -0.1)
+    ru, rpsi = oracle.ic2d_sums(np.ascontiguousarray(mesh.vert_x[:, sl].T),
+                                np.ascontiguousarray(mesh.face_x.T), z,
+                                mesh.face_area, mesh.face_mask,
+                                nthreads=min(16, os.cpu_count() or 1))
+    out = {
+        "workload": (f"Incompressible2D, icosahedral mesh level {level} "
+                     f"({mesh.n_verts} vertices + {mesh.n_leaves} faces), RH "
+                     f"5-4, Omega = 2 pi, eps = 0, RK2 dt = {dt}, FP64 velocity "
+                     f"+ stream function (log per pair)"),
+        "pairs_per_evaluation": pairs,
+        "evaluations_per_step": 2,
+        "value": round(2 * pairs * steps / el / 1e9, 3),
+        "unit": "Gpair-interactions/s",
+        "ms_per_step": round(el / steps * 1e3, 3),
+        "rk2_steps_per_s": round(steps / el, 4),
+        "dtype": "f64",
+        "graph": True,
+        "vs_analytic_rh54_t0_vertices_rel_l2_linf": errors(
+            u0, G.rh54_velocity(mesh.vert_x, 0.1, -0.1)),
+        "stream_fn_vs_analytic_rh54_t0_rel_l2_linf": errors1(
+            p0, G.rh54_stream_function(mesh.vert_x, 0.1, -0.1)),
+        "fp64_vs_cpu_oracle_sample": {
+            "targets": f"every {oracle_stride}th vertex",
+            "velocity_rel_l2_linf": errors(u0[:, sl], ru.T),
+            "stream_fn_rel_l2_linf": errors1(p0[sl], rpsi)},
+    }
+    log(f"incompressible2d: {el / steps * 1e3:.1f} ms/step, "
+        f"{out['value']:.1f} Gpairs/s")
+    return out
+
+
 def cpu_baseline(mesh, z_faces, budget_s=12.0):
     """Time the oracle (reference-faithful loop: every stored face, masked
     ones skipped) on a bounded sample of vertex targets."""
@@ -499,6 +564,8 @@ def main(argv=None):
                     help="skip the cubed-sphere mixed-vs-FP64 study")
     ap.add_argument("--no-swe", action="store_true",
                     help="skip the 16-state shallow-water ensemble")
+    ap.add_argument("--no-ic2d", action="store_true",
+                    help="skip the Incompressible2D run")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -635,6 +702,9 @@ def main(argv=None):
     if not a.no_swe:
         log("shallow-water ensemble (16 x level 6)")
         result["swe_ensemble"] = swe_ensemble_run(ctx)
+    if not a.no_ic2d:
+        log("Incompressible2D (level 6)")
+        result["incompressible2d"] = ic2d_run(ctx)
 
     if not a.no_cpu_baseline:
         log("CPU baseline (oracle) on a bounded sample")

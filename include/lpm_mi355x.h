@@ -358,6 +358,18 @@ int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
                      double Omega, double g, double eps, double pse_eps,
                      int precision, unsigned flags, void* stream);
 
+/* One RK2 (Heun) step in the stage layout of SWERK2::advance_timestep_impl
+ * (src/lpm_swe_rk2_impl.hpp:80-257): increments at the state, the work
+ * state = state + increments with its surface (SetSurfaceFromDepth /
+ * SetDepthAndSurfaceFromMassAndArea), sums and increments there, then
+ * state += increments1/2 + increments2/2 and the sums at the new state.
+ * The surface Laplacian comes from the PSE sums (pse_eps), where the
+ * reference evaluates a GMLS operator (Compadre).  Preconditions and flags
+ * as lpm_swe_rk4_step (LPM_RK4_REFERENCE_COMPAT has no effect). */
+int lpm_swe_rk2_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
+                     double Omega, double g, double eps, double pse_eps,
+                     int precision, unsigned flags, void* stream);
+
 /* Ensembles of sphere SWE problems stored back to back, as
  * lpm_bve_ensemble (same offset rules): one persistent launch evaluates the
  * sums of every member.  No reference counterpart; each member's results
@@ -374,6 +386,10 @@ int lpm_swe_ensemble_init_sums(lpm_ctx* ctx, const lpm_swe_ensemble* e,
                                double eps, double pse_eps, int precision,
                                unsigned flags, void* stream);
 int lpm_swe_ensemble_rk4_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                              double dt, double Omega, double g, double eps,
+                              double pse_eps, int precision, unsigned flags,
+                              void* stream);
+int lpm_swe_ensemble_rk2_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
                               double dt, double Omega, double g, double eps,
                               double pse_eps, int precision, unsigned flags,
                               void* stream);

@@ -156,9 +156,14 @@ _SIGS = {
                                   c_int, c_uint, c_vp]),
     "lpm_swe_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweFields), c_dbl, c_dbl,
                                  c_dbl, c_dbl, c_dbl, c_int, c_uint, c_vp]),
+    "lpm_swe_rk2_step": (c_int, [c_vp, ctypes.POINTER(SweFields), c_dbl, c_dbl,
+                                 c_dbl, c_dbl, c_dbl, c_int, c_uint, c_vp]),
     "lpm_swe_ensemble_init_sums": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
                                            c_dbl, c_dbl, c_int, c_uint, c_vp]),
     "lpm_swe_ensemble_rk4_step": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
+                                          c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+                                          c_int, c_uint, c_vp]),
+    "lpm_swe_ensemble_rk2_step": (c_int, [c_vp, ctypes.POINTER(SweEnsemble),
                                           c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                           c_int, c_uint, c_vp]),
     "lpm_ic2d_sums": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,

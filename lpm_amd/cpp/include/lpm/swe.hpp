@@ -192,4 +192,33 @@ class SWERK4 {
   unsigned flags_;
 };
 
+// Sphere RK2 (Heun) in the stage layout of SWERK2
+// (src/lpm_swe_rk2_impl.hpp:80-257); the surface Laplacian comes from the
+// PSE sums.  The reference's topo (zero bottom here) and GMLS parameters
+// have no counterpart.
+template <typename SeedType>
+class SWERK2 {
+ public:
+  SWERK2(const Real timestep, SWE<SeedType>& swe, int precision = LPM_FP64,
+         unsigned flags = 0)
+      : dt(timestep), t_idx(0), precision_(precision), flags_(flags) {
+    (void)swe;
+  }
+
+  void advance_timestep_impl(SWE<SeedType>& swe) {
+    lpm_swe_fields f = swe.fields();
+    check(lpm_swe_rk2_step(swe.ctx(), &f, dt, swe.coriolis.Omega, swe.g,
+                           swe.eps, swe.pse_eps, precision_, flags_, nullptr),
+          swe.ctx());
+    ++t_idx;
+  }
+
+  Real dt;
+  int t_idx;
+
+ private:
+  int precision_;
+  unsigned flags_;
+};
+
 }  // namespace Lpm

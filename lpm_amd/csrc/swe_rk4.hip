@@ -1,6 +1,9 @@
-// Sphere SWE entry points: direct sums and the RK4 stepper.  The stepper
+// Sphere SWE entry points: direct sums and the RK4 and RK2 steppers.  RK4
 // follows the stage layout of SWERK4::advance_timestep
-// (src/lpm_swe_rk4_impl.hpp:204-445) with the sphere sums and the sphere
+// (src/lpm_swe_rk4_impl.hpp:204-445), RK2 (Heun) that of
+// SWERK2::advance_timestep_impl (src/lpm_swe_rk2_impl.hpp:80-257) with the
+// surface Laplacian taken from the PSE sums in place of the reference's GMLS
+// evaluation; both use the sphere sums and the sphere
 // branch of the tendency functors (src/lpm_swe_kernels.hpp:941-1069), zero
 // bottom topography (SetSurfaceFromDepth / SetDepthAndSurfaceFromMassAndArea,
 // :1079-1140).  Per stage, one fused kernel per particle kind evaluates the
@@ -152,9 +155,10 @@ struct SweUpdateArgs {
   const uint8_t* mask;
   int64_t n;
   int is_face;
+  int order;  // 4: SWERK4Update weights; 2: Heun, x += (x1/2 + x2/2)
 };
 
-// SWERK4Update and the surface of the new state.
+// SWERK4Update (or the RK2 update) and the surface of the new state.
 __global__ __launch_bounds__(256) void swe_update_kernel(SweUpdateArgs A) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
@@ -162,6 +166,25 @@ __global__ __launch_bounds__(256) void swe_update_kernel(SweUpdateArgs A) {
   constexpr double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
   const int64_t n = A.n;
   double* x = A.x + i * A.xes;
+  if (A.order == 2) {
+    // KokkosBlas::update(0.5, k1, 0.5, k2, 1, state) of the reference's RK2
+    for (int c = 0; c < 3; ++c) {
+      const int64_t o = i + c * n;
+      x[c * A.xcs] += 0.5 * A.k[0][o] + 0.5 * A.k[1][o];
+    }
+    A.zeta[i] += 0.5 * A.zk[0][i] + 0.5 * A.zk[1][i];
+    A.sigma[i] += 0.5 * A.sk[0][i] + 0.5 * A.sk[1][i];
+    const double h = A.ha[i] + (0.5 * A.hk[0][i] + 0.5 * A.hk[1][i]);
+    A.ha[i] = h;
+    if (!A.is_face) {
+      A.surf[i] = h;
+    } else if (!(A.mask && A.mask[i])) {
+      const double d = A.mass[i] / h;
+      A.face_h[i] = d;
+      A.surf[i] = d;
+    }
+    return;
+  }
   for (int c = 0; c < 3; ++c) {
     const int64_t o = i + c * n;
     x[c * A.xcs] += sixth * (A.k[0][o] + A.k[3][o]) +
@@ -235,15 +258,20 @@ CrdIn swe_face_crd(const lpm_swe_fields& f) {
   return CrdIn{f.face_x, f.face_x_es, f.face_x_cs};
 }
 
+// One step of order 4 (RK4) or 2 (Heun RK2).  RK2: the work state is
+// state + stage-1 increments (KokkosBlas::update(1, x, 1, x1, 0, xwork)),
+// its sums and tendencies give the stage-2 increments.
 void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
-                    double Omega, double g, unsigned flags, SweStage& V,
-                    SweStage& F, const SweEvalFn& eval, hipStream_t s) {
+                    double Omega, double g, unsigned flags, int order,
+                    SweStage& V, SweStage& F, const SweEvalFn& eval,
+                    hipStream_t s) {
   const uint8_t* fskip =
       (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
-  const bool compat = (flags & LPM_RK4_REFERENCE_COMPAT) != 0;
+  const bool compat = order == 4 && (flags & LPM_RK4_REFERENCE_COMPAT) != 0;
   const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
   const double coef[3] = {0.5, 0.5, 1.0};
-  for (int stage = 0; stage < 4; ++stage) {
+  const int nstage = order == 2 ? 2 : 4;
+  for (int stage = 0; stage < nstage; ++stage) {
     if (stage > 0)
       eval(map, CrdIn{V.xw, 1, V.n}, CrdIn{F.xw, 1, F.n}, F.zw, F.sw, F.hw);
     for (int kind = 0; kind < 2; ++kind) {
@@ -282,12 +310,12 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
       a.zk = B.zk[stage];
       a.sk = B.sk[stage];
       a.hk = B.hk[stage];
-      if (stage < 3) {
+      if (stage < nstage - 1) {
         a.xw = B.xw;
         a.zw = B.zw;
         a.sw = B.sw;
         a.hw = B.hw;
-        a.a = coef[stage];
+        a.a = order == 2 ? 1.0 : coef[stage];
       }
       a.surf = face ? f.face_surf : f.vert_surf;
       a.face_h = f.face_h;
@@ -330,6 +358,7 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
     u.mask = face ? f.face_mask : nullptr;
     u.n = n;
     u.is_face = face ? 1 : 0;
+    u.order = order;
     swe_update_kernel<<<nblk(n), 256, 0, s>>>(u);
     LPM_HIP_CHECK(hipGetLastError());
   }
@@ -339,9 +368,10 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
 
 std::string swe_graph_key(const lpm_swe_fields& f, double dt, double Omega,
                           double g, double eps, double pse_eps, int precision,
-                          unsigned flags, hipStream_t s, uint64_t serial) {
+                          unsigned flags, int order, hipStream_t s,
+                          uint64_t serial) {
   std::ostringstream k;
-  k << "swe|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
+  k << "swe" << order << '|' << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
     << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_sigma << '|'
     << f.vert_h << '|' << f.vert_u << '|' << f.vert_u_es << '|'
     << f.vert_u_cs << '|' << f.vert_ddot << '|' << f.vert_laps << '|'
@@ -498,9 +528,14 @@ int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
   });
 }
 
-int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
-                     double Omega, double g, double eps, double pse_eps,
-                     int precision, unsigned flags, void* stream) {
+}  // extern "C"
+
+namespace lpm_amd {
+namespace {
+
+int swe_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt, double Omega,
+             double g, double eps, double pse_eps, int precision,
+             unsigned flags, int order, void* stream) {
   return guarded(ctx, [&] {
     check_step_args(ctx, precision, dt, Omega, g, eps, pse_eps);
     validate(f);
@@ -517,11 +552,59 @@ int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
           swe_pack_sources(ctx, swe_face_crd(*f), f->face_zeta, f->face_sigma,
                            f->face_area, f->face_surf, map, s);
           return swe_graph_key(*f, dt, Omega, g, eps, pse_eps, precision,
-                               flags, s, 0);
+                               flags, order, s, 0);
         },
-        [&]() { issue_swe_step(ctx, *f, dt, Omega, g, flags, V, F, ev, s); });
+        [&]() {
+          issue_swe_step(ctx, *f, dt, Omega, g, flags, order, V, F, ev, s);
+        });
     check_swe_outputs(ctx, *f, flags, s);
   });
+}
+
+int swe_ensemble_step(lpm_ctx* ctx, const lpm_swe_ensemble* e, double dt,
+                      double Omega, double g, double eps, double pse_eps,
+                      int precision, unsigned flags, int order, void* stream) {
+  return guarded(ctx, [&] {
+    check_step_args(ctx, precision, dt, Omega, g, eps, pse_eps);
+    const EnsembleDesc d = validate_swe_ensemble(e);
+    LPM_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = as_stream(stream);
+    const lpm_swe_fields& f = e->fields;
+    SweStage V = carve(ctx->swe_v, f.n_verts);
+    SweStage F = carve(ctx->swe_f, f.n_faces);
+    const SweEvalFn ev =
+        ensemble_swe_eval(ctx, d, f, eps, pse_eps, precision, flags, s);
+    run_maybe_graph(
+        ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
+        [&]() {
+          build_rank(ctx, f.face_mask, f.n_faces, s);
+          return swe_graph_key(f, dt, Omega, g, eps, pse_eps, precision, flags,
+                               order, s, swe_ensemble_prepare(ctx, d));
+        },
+        [&]() {
+          issue_swe_step(ctx, f, dt, Omega, g, flags, order, V, F, ev, s);
+        });
+    check_swe_outputs(ctx, f, flags, s);
+  });
+}
+
+}  // namespace
+}  // namespace lpm_amd
+
+extern "C" {
+
+int lpm_swe_rk4_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
+                     double Omega, double g, double eps, double pse_eps,
+                     int precision, unsigned flags, void* stream) {
+  return swe_step(ctx, f, dt, Omega, g, eps, pse_eps, precision, flags, 4,
+                  stream);
+}
+
+int lpm_swe_rk2_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt,
+                     double Omega, double g, double eps, double pse_eps,
+                     int precision, unsigned flags, void* stream) {
+  return swe_step(ctx, f, dt, Omega, g, eps, pse_eps, precision, flags, 2,
+                  stream);
 }
 
 int lpm_swe_ensemble_init_sums(lpm_ctx* ctx, const lpm_swe_ensemble* e,
@@ -548,26 +631,16 @@ int lpm_swe_ensemble_rk4_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
                               double dt, double Omega, double g, double eps,
                               double pse_eps, int precision, unsigned flags,
                               void* stream) {
-  return guarded(ctx, [&] {
-    check_step_args(ctx, precision, dt, Omega, g, eps, pse_eps);
-    const EnsembleDesc d = validate_swe_ensemble(e);
-    LPM_HIP_CHECK(hipSetDevice(ctx->device));
-    hipStream_t s = as_stream(stream);
-    const lpm_swe_fields& f = e->fields;
-    SweStage V = carve(ctx->swe_v, f.n_verts);
-    SweStage F = carve(ctx->swe_f, f.n_faces);
-    const SweEvalFn ev =
-        ensemble_swe_eval(ctx, d, f, eps, pse_eps, precision, flags, s);
-    run_maybe_graph(
-        ctx, (flags & LPM_RK4_USE_GRAPH) != 0, s,
-        [&]() {
-          build_rank(ctx, f.face_mask, f.n_faces, s);
-          return swe_graph_key(f, dt, Omega, g, eps, pse_eps, precision, flags,
-                               s, swe_ensemble_prepare(ctx, d));
-        },
-        [&]() { issue_swe_step(ctx, f, dt, Omega, g, flags, V, F, ev, s); });
-    check_swe_outputs(ctx, f, flags, s);
-  });
+  return swe_ensemble_step(ctx, e, dt, Omega, g, eps, pse_eps, precision,
+                           flags, 4, stream);
+}
+
+int lpm_swe_ensemble_rk2_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
+                              double dt, double Omega, double g, double eps,
+                              double pse_eps, int precision, unsigned flags,
+                              void* stream) {
+  return swe_ensemble_step(ctx, e, dt, Omega, g, eps, pse_eps, precision,
+                           flags, 2, stream);
 }
 
 }  // extern "C"

@@ -211,6 +211,14 @@ class Context:
             float(eps), float(pse_eps), precision_id(precision), flags,
             stream if stream is not None else current_stream()))
 
+    def swe_rk2_step(self, fields, dt, Omega, g, eps, pse_eps,
+                     precision="fp64", flags=0, stream=None):
+        """One sphere SWE RK2 (Heun) step, PSE surface Laplacian."""
+        self.check(self.L.lpm_swe_rk2_step(
+            self.h, ctypes.byref(fields), float(dt), float(Omega), float(g),
+            float(eps), float(pse_eps), precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
     def swe_ensemble_init_sums(self, ens, eps, pse_eps, precision="fp64",
                                flags=0, stream=None):
         self.check(self.L.lpm_swe_ensemble_init_sums(
@@ -221,6 +229,13 @@ class Context:
     def swe_ensemble_rk4_step(self, ens, dt, Omega, g, eps, pse_eps,
                               precision="fp64", flags=0, stream=None):
         self.check(self.L.lpm_swe_ensemble_rk4_step(
+            self.h, ctypes.byref(ens), float(dt), float(Omega), float(g),
+            float(eps), float(pse_eps), precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
+
+    def swe_ensemble_rk2_step(self, ens, dt, Omega, g, eps, pse_eps,
+                              precision="fp64", flags=0, stream=None):
+        self.check(self.L.lpm_swe_ensemble_rk2_step(
             self.h, ctypes.byref(ens), float(dt), float(Omega), float(g),
             float(eps), float(pse_eps), precision_id(precision), flags,
             stream if stream is not None else current_stream()))

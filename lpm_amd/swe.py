@@ -1,9 +1,11 @@
-"""SWE / SWERK4: the reference's sphere shallow-water API over the ABI.
+"""SWE / SWERK4 / SWERK2: the reference's sphere shallow-water API over the
+ABI.
 
 Mirrors ``SWE<Seed>`` (src/lpm_swe.hpp:18-177, src/lpm_swe_impl.hpp) for the
 sphere with zero bottom topography, and a sphere RK4 stepper in the stage
 layout of ``SWERK4`` (src/lpm_swe_rk4.hpp:8-126,
-src/lpm_swe_rk4_impl.hpp:204-445).  Fields are device tensors; the direct
+src/lpm_swe_rk4_impl.hpp:204-445), and the RK2 stepper of ``SWERK2``
+(src/lpm_swe_rk2_impl.hpp:80-257).  Fields are device tensors; the direct
 sums, tendencies and stage updates run in the library's HIP kernels.
 """
 from __future__ import annotations
@@ -149,6 +151,28 @@ class SWERK4:
         self.t_idx += 1
 
 
+class SWERK2(SWERK4):
+    """Sphere RK2 (Heun) for SWE in the stage layout of ``SWERK2``
+    (src/lpm_swe_rk2.hpp:17-60, src/lpm_swe_rk2_impl.hpp:80-257): 2 direct-sum
+    evaluations per step, the surface Laplacian from the PSE sums where the
+    reference evaluates GMLS.  The reference's ``topo`` (zero bottom here) and
+    ``gmls_params`` have no counterpart."""
+
+    def __init__(self, dt: float, swe: SWE, precision="fp64", leaf_only=False,
+                 use_graph=False, stream=None):
+        super().__init__(dt, swe, precision, reference_compat=False,
+                         leaf_only=leaf_only, use_graph=use_graph,
+                         stream=stream)
+
+    def advance_timestep_impl(self, swe: SWE):
+        if self._fields is None:
+            self._fields = swe.fields()
+        swe.ctx.swe_rk2_step(self._fields, self.dt, swe.Omega, swe.g, swe.eps,
+                             swe.pse_eps, self.precision, self.flags,
+                             self.stream)
+        self.t_idx += 1
+
+
 class SWEEnsemble(SWE):
     """Independent sphere SWE problems stored back to back (same field names
     as SWE, concatenated), evaluated by one persistent launch per direct sum
@@ -242,6 +266,18 @@ class SWEEnsembleRK4(SWERK4):
         if self._fields is None:
             self._fields = swe.descriptor()
         swe.ctx.swe_ensemble_rk4_step(self._fields, self.dt, swe.Omega, swe.g,
+                                      swe.eps, swe.pse_eps, self.precision,
+                                      self.flags, self.stream)
+        self.t_idx += 1
+
+
+class SWEEnsembleRK2(SWERK2):
+    """RK2 of every member of an SWEEnsemble (flags as SWERK2)."""
+
+    def advance_timestep_impl(self, swe: SWEEnsemble):
+        if self._fields is None:
+            self._fields = swe.descriptor()
+        swe.ctx.swe_ensemble_rk2_step(self._fields, self.dt, swe.Omega, swe.g,
                                       swe.eps, swe.pse_eps, self.precision,
                                       self.flags, self.stream)
         self.t_idx += 1

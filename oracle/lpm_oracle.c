@@ -684,6 +684,104 @@ void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
   free(fxw); free(fzw); free(fsw); free(faw);
 }
 
+/* One sphere SWE RK2 (Heun) step in the stage layout of
+ * SWERK2::advance_timestep_impl (src/lpm_swe_rk2_impl.hpp:80-257):
+ * increments at the state (positions dt u, tendencies), the work state
+ * state + increments (:104-121) with its surface (:122-131), the sums at the
+ * work state (:155-166) and increments there (:168-185), then
+ * state += increments1 / 2 + increments2 / 2 (:190-205), the surface and the
+ * sums at the new state (:207-232).  The surface Laplacian is the PSE sum
+ * (the reference evaluates a GMLS operator there).  Arguments as
+ * ora_swe_rk4_step.  Pinned by: steady TC2 state, second-order agreement with
+ * the RK4 step (error ratio under dt halving), and, for dt -> 0, the stage
+ * layout's first-order term. */
+void ora_swe_rk2_step(double* vx, double* vzeta, double* vsigma, double* vh,
+                      double* vu, double* vdd, double* vlap, double* vs,
+                      int64_t nv, double* fx, double* fzeta, double* fsigma,
+                      double* farea, const double* fmass, double* fh,
+                      double* fu, double* fdd, double* flap, double* fs,
+                      const uint8_t* fmask, int64_t nf, double dt,
+                      double Omega, double g, double eps, double pse_eps,
+                      int nthreads) {
+  const int64_t nvv = nv > 0 ? nv : 1, nff = nf > 0 ? nf : 1;
+  double* vx1 = (double*)calloc(3 * nvv, sizeof(double));
+  double* vx2 = (double*)calloc(3 * nvv, sizeof(double));
+  double* vxw = (double*)calloc(3 * nvv, sizeof(double));
+  double* fx1 = (double*)calloc(3 * nff, sizeof(double));
+  double* fx2 = (double*)calloc(3 * nff, sizeof(double));
+  double* fxw = (double*)calloc(3 * nff, sizeof(double));
+  /* per kind: zeta, sigma, h|A increments of stages 1, 2 and work values */
+  double* v1 = (double*)calloc(3 * nvv, sizeof(double));
+  double* v2 = (double*)calloc(3 * nvv, sizeof(double));
+  double* vw = (double*)calloc(3 * nvv, sizeof(double));
+  double* f1 = (double*)calloc(3 * nff, sizeof(double));
+  double* f2 = (double*)calloc(3 * nff, sizeof(double));
+  double* fw = (double*)calloc(3 * nff, sizeof(double));
+
+  /* stage 1 */
+  for (int64_t i = 0; i < 3 * nv; ++i) vx1[i] = dt * vu[i];
+  for (int64_t i = 0; i < 3 * nf; ++i) fx1[i] = dt * fu[i];
+  ora_swe_tendencies(vx, vu, vzeta, vsigma, vh, vdd, vlap, nv, Omega, g, dt, 0,
+                     v1, v1 + nvv, v1 + 2 * nvv);
+  ora_swe_tendencies(fx, fu, fzeta, fsigma, farea, fdd, flap, nf, Omega, g, dt,
+                     1, f1, f1 + nff, f1 + 2 * nff);
+  /* work state at t + dt */
+  for (int64_t i = 0; i < nv; ++i) {
+    for (int c = 0; c < 3; ++c) vxw[3 * i + c] = vx[3 * i + c] + vx1[3 * i + c];
+    vw[i] = vzeta[i] + v1[i];
+    vw[nvv + i] = vsigma[i] + v1[nvv + i];
+    vw[2 * nvv + i] = vh[i] + v1[2 * nvv + i];
+    vs[i] = vw[2 * nvv + i];
+  }
+  for (int64_t i = 0; i < nf; ++i) {
+    for (int c = 0; c < 3; ++c) fxw[3 * i + c] = fx[3 * i + c] + fx1[3 * i + c];
+    fw[i] = fzeta[i] + f1[i];
+    fw[nff + i] = fsigma[i] + f1[nff + i];
+    fw[2 * nff + i] = farea[i] + f1[2 * nff + i];
+    if (!(fmask && fmask[i])) {
+      fh[i] = fmass[i] / fw[2 * nff + i];
+      fs[i] = fh[i];
+    }
+  }
+  /* stage 2 */
+  ora_swe_sums(vxw, vs, nv, fxw, fw, fw + nff, fw + 2 * nff, fmask, fs, nf, 0,
+               eps, pse_eps, vu, vdd, vlap, nthreads);
+  ora_swe_sums(fxw, fs, nf, fxw, fw, fw + nff, fw + 2 * nff, fmask, fs, nf, 1,
+               eps, pse_eps, fu, fdd, flap, nthreads);
+  ora_swe_tendencies(vxw, vu, vw, vw + nvv, vw + 2 * nvv, vdd, vlap, nv, Omega,
+                     g, dt, 0, v2, v2 + nvv, v2 + 2 * nvv);
+  ora_swe_tendencies(fxw, fu, fw, fw + nff, fw + 2 * nff, fdd, flap, nf, Omega,
+                     g, dt, 1, f2, f2 + nff, f2 + 2 * nff);
+  for (int64_t i = 0; i < 3 * nv; ++i) vx2[i] = dt * vu[i];
+  for (int64_t i = 0; i < 3 * nf; ++i) fx2[i] = dt * fu[i];
+  /* update */
+  for (int64_t i = 0; i < nv; ++i) {
+    for (int c = 0; c < 3; ++c)
+      vx[3 * i + c] += 0.5 * vx1[3 * i + c] + 0.5 * vx2[3 * i + c];
+    vzeta[i] += 0.5 * v1[i] + 0.5 * v2[i];
+    vsigma[i] += 0.5 * v1[nvv + i] + 0.5 * v2[nvv + i];
+    vh[i] += 0.5 * v1[2 * nvv + i] + 0.5 * v2[2 * nvv + i];
+    vs[i] = vh[i];
+  }
+  for (int64_t i = 0; i < nf; ++i) {
+    for (int c = 0; c < 3; ++c)
+      fx[3 * i + c] += 0.5 * fx1[3 * i + c] + 0.5 * fx2[3 * i + c];
+    fzeta[i] += 0.5 * f1[i] + 0.5 * f2[i];
+    fsigma[i] += 0.5 * f1[nff + i] + 0.5 * f2[nff + i];
+    farea[i] += 0.5 * f1[2 * nff + i] + 0.5 * f2[2 * nff + i];
+    if (!(fmask && fmask[i])) {
+      fh[i] = fmass[i] / farea[i];
+      fs[i] = fh[i];
+    }
+  }
+  ora_swe_sums(vx, vs, nv, fx, fzeta, fsigma, farea, fmask, fs, nf, 0, eps,
+               pse_eps, vu, vdd, vlap, nthreads);
+  ora_swe_sums(fx, fs, nf, fx, fzeta, fsigma, farea, fmask, fs, nf, 1, eps,
+               pse_eps, fu, fdd, flap, nthreads);
+  free(vx1); free(vx2); free(vxw); free(fx1); free(fx2); free(fxw);
+  free(v1); free(v2); free(vw); free(f1); free(f2); free(fw);
+}
+
 /* ------------------------------------------------------------ galleries --
  * atan4: longitude in [0, 2 pi) (src/util/lpm_math.hpp:70-104). */
 static double atan4(double y, double x) {

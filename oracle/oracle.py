@@ -246,6 +246,20 @@ def swe_rk4_step(s: dict, dt, Omega, g, eps, pse_eps, compat=True,
     return s
 
 
+def swe_rk2_step(s: dict, dt, Omega, g, eps, pse_eps, nthreads=0):
+    """Advance the sphere SWE state ``s`` one RK2 (Heun) step."""
+    lib().ora_swe_rk2_step(
+        _d(s["vx"]), _d(s["vzeta"]), _d(s["vsigma"]), _d(s["vh"]),
+        _d(s["vu"]), _d(s["vdd"]), _d(s["vlap"]), _d(s["vs"]),
+        _i64(len(s["vx"])), _d(s["fx"]), _d(s["fzeta"]), _d(s["fsigma"]),
+        _d(s["farea"]), _d(s["fmass"]), _d(s["fh"]), _d(s["fu"]), _d(s["fdd"]),
+        _d(s["flap"]), _d(s["fs"]), _u8(s.get("fmask")), _i64(len(s["fx"])),
+        ctypes.c_double(dt), ctypes.c_double(Omega), ctypes.c_double(g),
+        ctypes.c_double(eps), ctypes.c_double(pse_eps),
+        ctypes.c_int(nthreads))
+    return s
+
+
 # ---------------------------------------------------------------- galleries
 def rh54_vorticity(x, u0, amp):
     x = _c3(x)

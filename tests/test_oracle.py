@@ -362,6 +362,122 @@ def test_swe_rk4_tc2_stays_steady(ora):
     assert rel_l2(s["fzeta"][leaf], z0[leaf]) < 2e-2
 
 
+def _swe_state(ora, m, noise=0.0, seed=3):
+    """TC2 state of mesh m (optionally with height noise) and its sums."""
+    from lpm_amd import galleries as G
+    rng = np.random.default_rng(seed)
+    s = {}
+    for kind, x in (("v", m.vert_x), ("f", m.face_x)):
+        st = G.tc2_state(x)
+        s[kind + "x"] = aos(x)
+        s[kind + "zeta"] = st["zeta"].copy()
+        s[kind + "sigma"] = st["sigma"].copy()
+        s[kind + "s"] = st["surface"] + noise * rng.uniform(-1, 1, x.shape[1])
+    s["vh"] = s["vs"].copy()
+    s["farea"] = m.face_area.copy()
+    s["fmask"] = m.face_mask.copy()
+    s["fh"] = s["fs"].copy()
+    s["fmass"] = s["fh"] * s["farea"]
+    h = m.mean_spacing()
+    u, dd, lap = ora.swe_sums(s["vx"], s["vs"], s["fx"], s["fzeta"], s["fsigma"],
+                              s["farea"], s["fs"], s["fmask"], eps=h,
+                              pse_eps=2 * h)
+    s["vu"], s["vdd"], s["vlap"] = u, dd, lap
+    u, dd, lap = ora.swe_sums(s["fx"], s["fs"], s["fx"], s["fzeta"], s["fsigma"],
+                              s["farea"], s["fs"], s["fmask"], collocated=True,
+                              eps=h, pse_eps=2 * h)
+    s["fu"], s["fdd"], s["flap"] = u, dd, lap
+    return s, h
+
+
+def test_swe_rk2_tc2_stays_steady(ora):
+    m = mesh(0, 3)
+    s, h = _swe_state(ora, m)
+    z0 = s["fzeta"].copy()
+    for _ in range(3):
+        ora.swe_rk2_step(s, 0.01, OMEGA, 1.0, h, 2 * h)
+    leaf = m.face_mask == 0
+    assert np.abs(s["fsigma"][leaf]).max() < 0.05
+    assert rel_l2(s["fzeta"][leaf], z0[leaf]) < 2e-2
+
+
+def test_swe_rk2_is_second_order(ora):
+    # a perturbed TC2 state integrated to t = 0.08: RK2 against RK4 at the
+    # finest step; halving dt must divide the RK2 error by ~4 (a forward
+    # Euler update, a stage evaluated at the wrong state or wrong weights
+    # give ~2)
+    m = mesh(0, 2)
+    T = 0.08
+    s4, h = _swe_state(ora, m, noise=0.05)
+    for _ in range(16):
+        ora.swe_rk4_step(s4, T / 16, OMEGA, 1.0, h, 2 * h, compat=False)
+    errs = []
+    for n in (4, 8, 16):
+        s, _ = _swe_state(ora, m, noise=0.05)
+        for _ in range(n):
+            ora.swe_rk2_step(s, T / n, OMEGA, 1.0, h, 2 * h)
+        errs.append(max(rel_l2(s["fsigma"], s4["fsigma"]),
+                        rel_l2(s["fx"], s4["fx"]),
+                        rel_l2(s["vh"], s4["vh"])))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.3 < r1 < 4.8 and 3.3 < r2 < 4.8, (errs, r1, r2)
+
+
+def test_swe_rk2_first_step_matches_hand_stage(ora):
+    # one step by hand from the oracle's sums and tendencies (no shared
+    # stepping code): work = state + k1, k2 at work, state + k1/2 + k2/2
+    m = mesh(1, 2)
+    s, h = _swe_state(ora, m, noise=0.05)
+    dt = 0.02
+    ref = {k: v.copy() for k, v in s.items()}
+    ora.swe_rk2_step(s, dt, OMEGA, 1.0, h, 2 * h)
+    f = ref
+    fx1 = dt * f["fu"]
+    t1 = _tend(f["fx"], f["fu"], f["fzeta"], f["fsigma"], f["farea"], f["fdd"],
+               f["flap"], dt, face=True)
+    vx1 = dt * f["vu"]
+    tv1 = _tend(f["vx"], f["vu"], f["vzeta"], f["vsigma"], f["vh"], f["vdd"],
+                f["vlap"], dt, face=False)
+    fxw, vxw = f["fx"] + fx1, f["vx"] + vx1
+    fw = [f["fzeta"] + t1[0], f["fsigma"] + t1[1], f["farea"] + t1[2]]
+    vw = [f["vzeta"] + tv1[0], f["vsigma"] + tv1[1], f["vh"] + tv1[2]]
+    # divided faces keep depth and surface
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fs = np.where(f["fmask"] != 0, f["fs"], f["fmass"] / fw[2])
+    vs = vw[2]
+    fu, fdd, flap = ora.swe_sums(fxw, fs, fxw, fw[0], fw[1], fw[2], fs,
+                                 f["fmask"], collocated=True, eps=h,
+                                 pse_eps=2 * h)
+    vu, vdd, vlap = ora.swe_sums(vxw, vs, fxw, fw[0], fw[1], fw[2], fs,
+                                 f["fmask"], eps=h, pse_eps=2 * h)
+    t2 = _tend(fxw, fu, fw[0], fw[1], fw[2], fdd, flap, dt, face=True)
+    tv2 = _tend(vxw, vu, vw[0], vw[1], vw[2], vdd, vlap, dt, face=False)
+    np.testing.assert_allclose(s["fx"], f["fx"] + 0.5 * fx1 + 0.5 * dt * fu,
+                               rtol=0, atol=1e-15)
+    np.testing.assert_allclose(s["vx"], f["vx"] + 0.5 * vx1 + 0.5 * dt * vu,
+                               rtol=0, atol=1e-15)
+    for got, x0, a, b in ((s["fzeta"], f["fzeta"], t1[0], t2[0]),
+                          (s["fsigma"], f["fsigma"], t1[1], t2[1]),
+                          (s["farea"], f["farea"], t1[2], t2[2]),
+                          (s["vzeta"], f["vzeta"], tv1[0], tv2[0]),
+                          (s["vsigma"], f["vsigma"], tv1[1], tv2[1]),
+                          (s["vh"], f["vh"], tv1[2], tv2[2])):
+        np.testing.assert_allclose(got, x0 + 0.5 * a + 0.5 * b, rtol=1e-13,
+                                   atol=1e-15)
+
+
+def _tend(x, u, zeta, sigma, ha, ddot, laps, dt, face, g=1.0):
+    """Sphere SWE tendencies in numpy (src/lpm_swe_kernels.hpp:941-1069 with
+    CoriolisSphere, src/lpm_coriolis.hpp:154-195)."""
+    f = 2 * OMEGA * x[:, 2]
+    dfdt = 2 * OMEGA * u[:, 2]
+    gfxu = -2 * OMEGA * (-u[:, 0] * x[:, 1] + u[:, 1] * x[:, 0])
+    dz = (-dfdt - (zeta + f) * sigma) * dt
+    ds = (f * zeta + gfxu - ddot - g * laps - (u * u).sum(1)) * dt
+    dh = (sigma * ha) * dt if face else (-sigma * ha) * dt
+    return dz, ds, dh
+
+
 def test_err_norms(ora):
     a = np.array([1.0, 2.0, 3.0])
     b = np.array([1.0, 2.5, 2.0])

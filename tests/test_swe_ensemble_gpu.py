@@ -122,9 +122,15 @@ def test_fp64_ensemble_matches_oracle(ctx, ora):
         assert rel_l2(g(ens.surf_lap_passive)[vs], rlap) < TOL64
 
 
+def _steppers(order):
+    from lpm_amd.swe import SWEEnsembleRK2, SWEEnsembleRK4, SWERK2, SWERK4
+    return (SWEEnsembleRK4, SWERK4) if order == 4 else (SWEEnsembleRK2, SWERK2)
+
+
+@pytest.mark.parametrize("order", [4, 2])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_rk4_equals_members_alone(ctx, precision):
-    from lpm_amd.swe import SWEEnsembleRK4, SWERK4
+def test_rk_equals_members_alone(ctx, precision, order):
+    SWEEnsembleRK4, SWERK4 = _steppers(order)
     skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
     ens = _ensemble(ctx)
     ens.init_direct_sums(precision, skip)
@@ -143,9 +149,10 @@ def test_rk4_equals_members_alone(ctx, precision):
                       leaf_only=True)
 
 
+@pytest.mark.parametrize("order", [4, 2])
 @pytest.mark.parametrize("ensemble", [False, True])
-def test_graph_replays_match_eager(ctx, ensemble):
-    from lpm_amd.swe import SWEEnsembleRK4, SWERK4
+def test_graph_replays_match_eager(ctx, ensemble, order):
+    SWEEnsembleRK4, SWERK4 = _steppers(order)
     skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
     if ensemble:
         a, b = _ensemble(ctx), _ensemble(ctx)

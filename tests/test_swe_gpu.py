@@ -158,6 +158,26 @@ def test_swe_rk4_matches_oracle(ctx, ora, seed, compat):
         assert rel_l2(got[k][leaf], st[k][leaf]) < 1e-11, k
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_swe_rk2_matches_oracle(ctx, ora, seed):
+    from lpm_amd.swe import SWERK2
+    s = _swe(ctx, seed, 2)
+    s.init_direct_sums()
+    torch.cuda.synchronize()
+    st = _oracle_state(s)
+    leaf = st["fmask"] == 0
+    solver = SWERK2(0.01, s)
+    for _ in range(3):
+        s.advance_timestep(solver)
+        ora.swe_rk2_step(st, 0.01, OMEGA, s.g, s.eps, s.pse_eps)
+    torch.cuda.synchronize()
+    got = _oracle_state(s)
+    for k in ("vx", "vzeta", "vsigma", "vh", "vu", "vdd", "vlap"):
+        assert rel_l2(got[k], st[k]) < 1e-11, k
+    for k in ("fx", "fzeta", "fsigma", "farea", "fu", "fdd", "flap", "fh"):
+        assert rel_l2(got[k][leaf], st[k][leaf]) < 1e-11, k
+
+
 def test_swe_argument_errors(ctx):
     L = _abi.lib()
     x = dev(np.eye(3))
