@@ -178,11 +178,18 @@ class PolyMesh2d {
     check(lpm_mesh_copy_vertices(m, vx.data()));
     check(lpm_mesh_copy_faces(m, fx.data(), face_area.host().data(),
                               face_mask.host().data(), nullptr));
+    int nfv = 0;
+    check(lpm_mesh_copy_face_vertices(m, nullptr, &nfv));
+    face_verts.assign(static_cast<size_t>(nf) * nfv, 0);
+    check(lpm_mesh_copy_face_vertices(m, face_verts.data(), &nfv));
+    nfaceverts = nfv;
     lpm_mesh_destroy(m);
     for (int k = 0; k < 3; ++k)
       for (int64_t i = 0; i < nv; ++i) vert_x.host(i, k) = vx[k * nv + i];
     for (int k = 0; k < 3; ++k)
       for (int64_t i = 0; i < nf; ++i) face_x.host(i, k) = fx[k * nf + i];
+    vert_lag = std::move(vx);
+    face_lag = std::move(fx);
     init_fields();
     PolyMesh2d::update_device();
   }
@@ -210,6 +217,11 @@ class PolyMesh2d {
   VectorField face_x;    // faces.phys_crds
   ScalarField face_area; // faces.area
   DualArray<uint8_t> face_mask;
+  // host only: face corners [n_faces][nfaceverts] (faces.verts) and the
+  // initial positions, component-major [3][n] (vertices/faces.lag_crds)
+  std::vector<int32_t> face_verts;
+  int nfaceverts = 0;
+  std::vector<Real> vert_lag, face_lag;
 
  protected:
   virtual void init_fields() {}

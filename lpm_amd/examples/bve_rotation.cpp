@@ -3,7 +3,11 @@
 // Omega_r = 2 pi, background rotation set to 0, RK4 to tf; reports velocity,
 // stream-function and position errors against the exact rotation as JSON.
 //   bve_rotation [-d depth] [-dt dt] [-tf tfinal] [-seed cube|icos]
-//                [-precision fp64|mixed] [-renorm 0|1]
+//                [-precision fp64|mixed] [-renorm 0|1] [-o case_name]
+//                [-f output_interval]
+// -f n > 0 writes .vtp frames (vtk_interface) at t = 0, every n steps and at
+// the end, named case_name_<seed><depth>_dt<dt>_<frame>.vtp, as the
+// reference's driver does.
 // -renorm 1 projects RK4 stage positions back onto the sphere.  Without it
 // (the reference's stepper) stage positions sit off the sphere by
 // O((dt |u|)^2), which shifts 1 - x.y of nearest neighbours by the same
@@ -15,6 +19,7 @@
 #include <string>
 
 #include "lpm/bve_sphere.hpp"
+#include "lpm/vtk.hpp"
 
 using namespace Lpm;
 
@@ -27,6 +32,8 @@ struct Args {
   std::string seed = "cube";
   int precision = LPM_FP64;
   bool renorm = false;
+  std::string case_name = "bve_rotation";
+  int output_interval = 0;
 };
 
 Args parse(int argc, char** argv) {
@@ -39,6 +46,8 @@ Args parse(int argc, char** argv) {
     else if (k == "-seed") a.seed = v;
     else if (k == "-precision") a.precision = v == "mixed" ? LPM_MIXED : LPM_FP64;
     else if (k == "-renorm") a.renorm = std::stoi(v) != 0;
+    else if (k == "-o") a.case_name = v;
+    else if (k == "-f") a.output_interval = std::stoi(v);
     else throw std::runtime_error("unknown option " + k);
   }
   return a;
@@ -82,7 +91,22 @@ int run(const Args& a) {
   const unsigned flags =
       LPM_RK4_REFERENCE_COMPAT | (a.renorm ? LPM_RK4_RENORMALIZE : 0u);
   BVERK4 solver(a.dt, sphere, a.precision, flags);
-  for (int s = 0; s < nsteps; ++s) solver.advance_timestep(sphere);
+  int frame = 0;
+  auto output = [&]() {
+    char name[256];
+    std::snprintf(name, sizeof name, "%s_%s%d_dt%g_%04d.vtp",
+                  a.case_name.c_str(), Seed::id_string().c_str(), a.depth, a.dt,
+                  frame++);
+    sphere.update_host();
+    vtk_interface(sphere).write(name);
+  };
+  if (a.output_interval > 0) output();
+  for (int s = 0; s < nsteps; ++s) {
+    solver.advance_timestep(sphere);
+    if (a.output_interval > 0 &&
+        ((s + 1) % a.output_interval == 0 || s + 1 == nsteps))
+      output();
+  }
   sphere.update_host();
   const Real ang = OMG * sphere.t;
   Real pos_err = 0;

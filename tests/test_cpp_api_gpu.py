@@ -12,11 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "lpm_amd", "_build")
 
 
-def run(name, *args):
+def run(name, *args, cwd=None):
     exe = os.path.join(BUILD, name)
     assert os.path.exists(exe), f"{exe} not built (python -m lpm_amd.build)"
     r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True,
-                       timeout=120)
+                       timeout=120, cwd=cwd)
     assert r.returncode == 0, r.stderr
     return json.loads(r.stdout.strip().splitlines()[-1])
 
@@ -52,3 +52,35 @@ def test_sphere_swe_tc2_driver():
     assert out["ddot_rel_l2"] < 7e-2
     assert out["laps_rel_l2"] < 5e-2
     assert out["max_abs_divergence"] < 0.05
+
+
+def test_bve_rotation_vtk_frames(tmp_path):
+    import numpy as np
+    from lpm_amd.mesh import SphereMesh
+    from lpm_amd.vtk import VtkPolymeshInterface, read_vtp
+    out = run("bve_rotation", "-d", 3, "-dt", 0.01, "-tf", 0.03, "-seed",
+              "icos", "-o", "rot", "-f", 2, cwd=tmp_path)
+    frames = sorted(p.name for p in tmp_path.glob("rot_*.vtp"))
+    # t = 0, after step 2 and after the last step (3)
+    assert len(frames) == 3, frames
+    f0 = read_vtp(str(tmp_path / frames[0]))
+    f2 = read_vtp(str(tmp_path / frames[-1]))
+    # the C++ writer's polygons and areas are the Python writer's
+    py = VtkPolymeshInterface(SphereMesh.build("icos", 3))
+    np.testing.assert_array_equal(f0["connectivity"], py.polys.reshape(-1))
+    np.testing.assert_array_equal(f0["cell_data"]["area"], py.cell_data["area"])
+    np.testing.assert_array_equal(f0["points"], f0["point_data"]["lag_crds"])
+    np.testing.assert_array_equal(f2["point_data"]["lag_crds"],
+                                  f0["point_data"]["lag_crds"])
+    x = f0["points"]
+    omg = 2 * np.pi
+    exact = np.stack([-omg * x[:, 1], omg * x[:, 0], 0 * x[:, 0]], 1)
+    u = f0["point_data"]["velocity"]
+    assert np.linalg.norm(u - exact) / np.linalg.norm(exact) < 2e-2
+    # the last frame holds the rotated positions the driver reports
+    ang = omg * out["t"]
+    c, s = np.cos(ang), np.sin(ang)
+    xr = np.stack([c * x[:, 0] - s * x[:, 1], s * x[:, 0] + c * x[:, 1],
+                   x[:, 2]], 1)
+    err = np.linalg.norm(f2["points"] - xr, axis=1).max()
+    assert err == pytest.approx(out["position_err_linf"], rel=1e-5)
