@@ -104,6 +104,24 @@ int lpm_mesh_copy_faces(const lpm_mesh* mesh, double* xyz_host,
 /* Face corner vertex indices, [n_faces][verts_per_face] row-major. */
 int lpm_mesh_copy_face_vertices(const lpm_mesh* mesh, int32_t* verts_host,
                                 int* verts_per_face);
+/* Face tree levels, 0 for seed faces (Faces::level,
+ * src/mesh/lpm_faces.hpp:147). */
+int lpm_mesh_copy_face_levels(const lpm_mesh* mesh, int32_t* level_host);
+/* Adaptive refinement, PolyMesh2d::divide_flagged_faces
+ * (src/mesh/lpm_polymesh2d_impl.hpp:129-173): in index order, every face
+ * i < n_faces with flags_host[i] != 0 whose level is <= depth + amr_limit is
+ * divided (a flagged face that is already divided stays as it is,
+ * PolyMesh2d::divide_face :46-53), new vertices and faces are appended and
+ * the leaf count is updated.  n_flags must equal n_faces.  max_faces > 0
+ * plays the reference's allocation limit: if the flag count exceeds
+ * (max_faces - n_faces) / 4 nothing is divided and *n_divided = -1.
+ * Otherwise *n_divided is the reference's refine count (flagged faces within
+ * the level limit) and *limit_reached != 0 if a flagged face was beyond it.
+ * Copy the arrays again afterwards (sizes grow). */
+int lpm_mesh_divide_flagged_faces(lpm_mesh* mesh, const uint8_t* flags_host,
+                                  int64_t n_flags, int amr_limit,
+                                  int64_t max_faces, int64_t* n_divided,
+                                  int* limit_reached);
 
 /* --------------------------------------------------------- BVE sums ---- */
 /* Velocity of the barotropic vorticity equation on the unit sphere by the

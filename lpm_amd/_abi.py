@@ -131,6 +131,9 @@ _SIGS = {
     "lpm_mesh_copy_vertices": (c_int, [c_vp, c_vp]),
     "lpm_mesh_copy_faces": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "lpm_mesh_copy_face_vertices": (c_int, [c_vp, c_vp, c_intp]),
+    "lpm_mesh_copy_face_levels": (c_int, [c_vp, c_vp]),
+    "lpm_mesh_divide_flagged_faces": (c_int, [c_vp, c_vp, c_i64, c_int, c_i64,
+                                              ctypes.POINTER(c_i64), c_intp]),
     "lpm_bve_velocity": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
                                  c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_dbl,
                                  c_int, c_uint, c_vp, c_i64, c_i64, c_vp]),

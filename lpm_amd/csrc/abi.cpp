@@ -250,6 +250,40 @@ int lpm_mesh_copy_face_vertices(const lpm_mesh* mesh, int32_t* verts,
   });
 }
 
+int lpm_mesh_copy_face_levels(const lpm_mesh* mesh, int32_t* level) {
+  return guarded(nullptr, [&] {
+    LPM_ARG_CHECK(mesh && mesh->m, "null mesh");
+    const auto& m = *mesh->m;
+    if (level)
+      std::memcpy(level, m.flevel().data(), sizeof(int32_t) * m.n_faces());
+  });
+}
+
+int lpm_mesh_divide_flagged_faces(lpm_mesh* mesh, const uint8_t* flags,
+                                  int64_t n_flags, int amr_limit,
+                                  int64_t max_faces, int64_t* n_divided,
+                                  int* limit_reached) {
+  return guarded(nullptr, [&] {
+    LPM_ARG_CHECK(mesh && mesh->m, "null mesh");
+    auto& m = *mesh->m;
+    LPM_ARG_CHECK(n_flags == m.n_faces(),
+                  "lpm_mesh_divide_flagged_faces: n_flags must equal n_faces");
+    LPM_ARG_CHECK(flags || n_flags == 0, "flags is NULL");
+    LPM_ARG_CHECK(amr_limit >= 0, "amr_limit must be >= 0");
+    int64_t flagged = 0;
+    for (int64_t i = 0; i < n_flags; ++i) flagged += flags[i] ? 1 : 0;
+    if (limit_reached) *limit_reached = 0;
+    if (max_faces > 0 && flagged > (max_faces - m.n_faces()) / 4) {
+      if (n_divided) *n_divided = -1;
+      return;
+    }
+    bool lim = false;
+    const int64_t c = m.divide_flagged(flags, m.depth() + amr_limit, &lim);
+    if (n_divided) *n_divided = c;
+    if (limit_reached) *limit_reached = lim ? 1 : 0;
+  });
+}
+
 // ------------------------------------------------------------ BVE sums
 static void check_crd(const double* p, int64_t es, int64_t cs, int64_t n,
                       const char* what) {

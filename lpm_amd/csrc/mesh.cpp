@@ -212,6 +212,7 @@ SphereTreeMesh::SphereTreeMesh(SeedKind kind, int depth)
   fmask_.reserve(nf);
   fkid_.reserve(nf);
   fparent_.reserve(nf);
+  flevel_.reserve(nf);
   fverts_.reserve(nf * nfv_);
   fedges_.reserve(nf * nfv_);
 
@@ -284,6 +285,7 @@ void SphereTreeMesh::add_face(const int32_t* verts, const int32_t* edges,
   fmask_.push_back(0);
   fkid_.push_back(-1);
   fparent_.push_back(parent);
+  flevel_.push_back(parent < 0 ? 0 : flevel_[parent] + 1);
   for (int k = 0; k < nfv_; ++k) {
     fverts_.push_back(verts[k]);
     fedges_.push_back(edges[k]);
@@ -428,6 +430,35 @@ void SphereTreeMesh::divide_quad(int32_t f) {
   farea_[f] = 0.0;
   fmask_[f] = 1;
   --n_leaves_;
+}
+
+bool SphereTreeMesh::divide_face(int32_t f) {
+  if (f < 0 || f >= n_faces())
+    throw std::out_of_range("SphereTreeMesh::divide_face: bad face index");
+  if (fkid_[f] >= 0) return false;
+  if (nfv_ == 3)
+    divide_tri(f);
+  else
+    divide_quad(f);
+  return true;
+}
+
+int64_t SphereTreeMesh::divide_flagged(const uint8_t* flags, int max_level,
+                                       bool* limit_reached) {
+  const int64_t n_in = n_faces();
+  int64_t count = 0;
+  bool limit = false;
+  for (int64_t i = 0; i < n_in; ++i) {
+    if (!flags[i]) continue;
+    if (flevel_[i] <= max_level) {
+      divide_face(static_cast<int32_t>(i));
+      ++count;
+    } else {
+      limit = true;
+    }
+  }
+  if (limit_reached) *limit_reached = limit;
+  return count;
 }
 
 double SphereTreeMesh::surface_area() const {

@@ -35,8 +35,17 @@ class SphereTreeMesh {
   int64_t n_faces() const { return static_cast<int64_t>(fx_.size()); }
   int64_t n_edges() const { return static_cast<int64_t>(eo_.size()); }
   int64_t n_leaves() const { return n_leaves_; }
-  // Leaves of a uniform refinement are the last n_leaves() faces.
-  int64_t first_leaf() const { return n_faces() - n_leaves_; }
+
+  // Divides leaf face f (PolyMesh2d::divide_face,
+  // src/mesh/lpm_polymesh2d_impl.hpp:46-53); false if f is already divided.
+  bool divide_face(int32_t f);
+  // PolyMesh2d::divide_flagged_faces (src/mesh/lpm_polymesh2d_impl.hpp:
+  // 129-173): faces i < n_faces() with flags[i] != 0, in index order, whose
+  // level is <= max_level.  Returns the reference's refine count (flagged
+  // faces within the level limit); *limit_reached is set when a flagged face
+  // was beyond it.
+  int64_t divide_flagged(const uint8_t* flags, int max_level,
+                         bool* limit_reached);
 
   const std::vector<double>& vx() const { return vx_; }
   const std::vector<double>& vy() const { return vy_; }
@@ -48,6 +57,7 @@ class SphereTreeMesh {
   const std::vector<uint8_t>& fmask() const { return fmask_; }
   const std::vector<int32_t>& fverts() const { return fverts_; }
   const std::vector<int32_t>& fparent() const { return fparent_; }
+  const std::vector<int32_t>& flevel() const { return flevel_; }
 
   double surface_area() const;  // sum over all stored faces (divided = 0)
 
@@ -70,7 +80,7 @@ class SphereTreeMesh {
   std::vector<int32_t> eo_, ed_, el_, er_, ek_;  // ek_: first kid or -1
   std::vector<double> fx_, fy_, fz_, farea_;
   std::vector<uint8_t> fmask_;
-  std::vector<int32_t> fverts_, fedges_, fkid_, fparent_;
+  std::vector<int32_t> fverts_, fedges_, fkid_, fparent_, flevel_;
 };
 
 // Sphere geometry helpers in the operation order of SphereGeometry

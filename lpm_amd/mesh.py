@@ -48,32 +48,12 @@ class SphereMesh:
     n_leaves: int
     n_edges: int
 
+    face_level: np.ndarray = None  # (n_faces,) int32, 0 = seed face
+
     @classmethod
     def build(cls, seed, depth: int) -> "SphereMesh":
-        L = _abi.lib()
-        h = ctypes.c_void_p()
-        _abi.check(L.lpm_mesh_create(seed_id(seed), depth, ctypes.byref(h)))
-        try:
-            nv, nf, nl, ne = (ctypes.c_int64() for _ in range(4))
-            _abi.check(L.lpm_mesh_sizes(h, ctypes.byref(nv), ctypes.byref(nf),
-                                        ctypes.byref(nl), ctypes.byref(ne)))
-            vx = np.empty((3, nv.value))
-            fx = np.empty((3, nf.value))
-            fa = np.empty(nf.value)
-            fm = np.empty(nf.value, np.uint8)
-            fp = np.empty(nf.value, np.int32)
-            _abi.check(L.lpm_mesh_copy_vertices(h, vx.ctypes.data))
-            _abi.check(L.lpm_mesh_copy_faces(h, fx.ctypes.data, fa.ctypes.data,
-                                             fm.ctypes.data, fp.ctypes.data))
-            nfv = ctypes.c_int()
-            _abi.check(L.lpm_mesh_copy_face_vertices(h, None, ctypes.byref(nfv)))
-            fv = np.empty((nf.value, nfv.value), np.int32)
-            _abi.check(L.lpm_mesh_copy_face_vertices(h, fv.ctypes.data,
-                                                     ctypes.byref(nfv)))
-        finally:
-            L.lpm_mesh_destroy(h)
-        return cls(seed_id(seed), depth, vx, fx, fa, fm, fp, fv, nl.value,
-                   ne.value)
+        """Uniform refinement to ``depth`` (PolyMesh2d::tree_init)."""
+        return PolyMesh2d(seed, depth).sphere_mesh()
 
     @property
     def n_verts(self) -> int:
@@ -85,7 +65,8 @@ class SphereMesh:
 
     @property
     def leaf_slice(self) -> slice:
-        """Leaves of a uniform refinement are the last n_leaves faces."""
+        """Leaves of a uniform refinement are the last n_leaves faces (not
+        so after adaptive refinement: use face_mask == 0)."""
         return slice(self.n_faces - self.n_leaves, self.n_faces)
 
     def useful_pairs(self) -> int:
@@ -95,3 +76,72 @@ class SphereMesh:
     def mean_spacing(self) -> float:
         """sqrt(4 pi / N_F), the mean leaf-panel size in radians."""
         return float(np.sqrt(4 * np.pi / self.n_leaves))
+
+
+class PolyMesh2d:
+    """The library's host quadtree mesh (PolyMesh2d<Seed>,
+    src/mesh/lpm_polymesh2d.hpp, src/mesh/lpm_polymesh2d_impl.hpp):
+    uniform refinement to ``depth`` (tree_init), then optional adaptive
+    refinement (``divide_flagged_faces``); ``sphere_mesh()`` copies the
+    arrays out.  ``amr_limit`` and ``max_faces`` play the reference's
+    PolyMeshParameters amr_limit and nmaxfaces (0: no allocation limit)."""
+
+    def __init__(self, seed, depth: int, amr_limit: int = 0,
+                 max_faces: int = 0):
+        self.seed = seed_id(seed)
+        self.depth = int(depth)
+        self.amr_limit = int(amr_limit)
+        self.max_faces = int(max_faces)
+        self._L = _abi.lib()
+        self._h = ctypes.c_void_p()
+        _abi.check(self._L.lpm_mesh_create(self.seed, self.depth,
+                                           ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.lpm_mesh_destroy(h)
+            self._h = None
+
+    def sizes(self):
+        """(n_verts, n_faces, n_leaves, n_edges)."""
+        v = [ctypes.c_int64() for _ in range(4)]
+        _abi.check(self._L.lpm_mesh_sizes(self._h, *map(ctypes.byref, v)))
+        return tuple(x.value for x in v)
+
+    def n_faces_host(self) -> int:
+        return self.sizes()[1]
+
+    def divide_flagged_faces(self, flags):
+        """Divide the flagged faces (one flag per stored face, index order,
+        within the level limit depth + amr_limit).  Returns (refine count,
+        limit reached); refine count -1: not enough room under max_faces,
+        nothing divided."""
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        n = ctypes.c_int64()
+        lim = ctypes.c_int()
+        _abi.check(self._L.lpm_mesh_divide_flagged_faces(
+            self._h, flags.ctypes.data, len(flags), self.amr_limit,
+            self.max_faces, ctypes.byref(n), ctypes.byref(lim)))
+        return n.value, bool(lim.value)
+
+    def sphere_mesh(self) -> SphereMesh:
+        L, h = self._L, self._h
+        nv, nf, nl, ne = self.sizes()
+        vx = np.empty((3, nv))
+        fx = np.empty((3, nf))
+        fa = np.empty(nf)
+        fm = np.empty(nf, np.uint8)
+        fp = np.empty(nf, np.int32)
+        fl = np.empty(nf, np.int32)
+        _abi.check(L.lpm_mesh_copy_vertices(h, vx.ctypes.data))
+        _abi.check(L.lpm_mesh_copy_faces(h, fx.ctypes.data, fa.ctypes.data,
+                                         fm.ctypes.data, fp.ctypes.data))
+        _abi.check(L.lpm_mesh_copy_face_levels(h, fl.ctypes.data))
+        nfv = ctypes.c_int()
+        _abi.check(L.lpm_mesh_copy_face_vertices(h, None, ctypes.byref(nfv)))
+        fv = np.empty((nf, nfv.value), np.int32)
+        _abi.check(L.lpm_mesh_copy_face_vertices(h, fv.ctypes.data,
+                                                 ctypes.byref(nfv)))
+        return SphereMesh(self.seed, self.depth, vx, fx, fa, fm, fp, fv, nl,
+                          ne, fl)

@@ -319,6 +319,40 @@ def test_every_variant_matches_oracle(ctx, ora, kid, prec):
         ctx.set_variant(kid, -1)
 
 
+@pytest.mark.parametrize("kid,prec", [(_abi.LPM_KERNEL_BVE_VEL_F64, "fp64"),
+                                      (_abi.LPM_KERNEL_BVE_VEL_MIXED, "mixed")])
+def test_adaptively_refined_mesh_matches_oracle(ctx, ora, kid, prec):
+    # a locally refined mesh (cap refined twice): leaves of three levels,
+    # divided faces interleaved with leaves in the face order
+    from lpm_amd.mesh import PolyMesh2d
+    pm = PolyMesh2d("cube", 3, amr_limit=2)
+    for _ in range(2):
+        m = pm.sphere_mesh()
+        pm.divide_flagged_faces((m.face_mask == 0) & (m.face_x[2] > 0.5))
+    m = pm.sphere_mesh()
+    assert len(np.unique(m.face_level[m.face_mask == 0])) == 3
+    z = rh(m.face_x)
+    vx, fx = dev(m.vert_x), dev(m.face_x)
+    args = (dev(z), dev(m.face_area), dev(m.face_mask, torch.uint8))
+    ref_v = ora.bve_velocity(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask)
+    ref_f = ora.bve_velocity(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask, collocated=True)
+    leaf = m.face_mask == 0
+    tol = TOL64 if prec == "fp64" else TOL_MIXED
+    names = ctx.variants(kid)
+    try:
+        for v in range(len(names)):
+            ctx.set_variant(kid, v)
+            uv = ctx.bve_velocity(vx, fx, *args, precision=prec).cpu().numpy()
+            uf = ctx.bve_velocity(fx, fx, *args, collocated=True,
+                                  precision=prec).cpu().numpy()
+            assert rel_l2(uv.T, ref_v) < tol, names[v]
+            assert rel_l2(uf.T[leaf], ref_f[leaf]) < tol, names[v]
+    finally:
+        ctx.set_variant(kid, -1)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_matrix_core_variant_on_a_larger_mesh(ctx, seed):
     # level 5: most source tiles are far from every wave, so most pairs take
