@@ -63,11 +63,22 @@ __device__ __forceinline__ double rcp_nr(double d) {
 }
 
 // One source: S += s / (1 - x.y [+ eps^2]) * y.
-template <bool EPS, int NR>
+// FUSE: 1 - x.y as one FMA chain from 1 (three operations instead of four;
+// the same O(1e-16) absolute rounding as the dot-then-subtract form).
+template <bool FUSE>
+__device__ __forceinline__ double one_minus_dot(const double* x,
+                                                const double4& y) {
+  if (FUSE)
+    return __builtin_fma(-x[2], y.z,
+                         __builtin_fma(-x[1], y.y, __builtin_fma(-x[0], y.x, 1.0)));
+  const double t = x[0] * y.x + x[1] * y.y + x[2] * y.z;
+  return 1.0 - t;
+}
+
+template <bool EPS, int NR, bool FUSE = false>
 __device__ __forceinline__ void f64_one(const double* x, const double4& y,
                                         double eps2, double* S) {
-  const double t = x[0] * y.x + x[1] * y.y + x[2] * y.z;
-  double d = 1.0 - t;
+  double d = one_minus_dot<FUSE>(x, y);
   if (EPS) d += eps2;
   const double w = y.w * rcp_nr<NR>(d);
   S[0] = __builtin_fma(w, y.x, S[0]);
@@ -77,13 +88,11 @@ __device__ __forceinline__ void f64_one(const double* x, const double4& y,
 
 // Two sources sharing one reciprocal: r = 1/(da db), 1/da = db r,
 // 1/db = da r.  Terms are accumulated in source order.
-template <bool EPS, int NR>
+template <bool EPS, int NR, bool FUSE = false>
 __device__ __forceinline__ void f64_two(const double* x, const double4& ya,
                                         const double4& yb, double eps2,
                                         double* S) {
-  const double ta = x[0] * ya.x + x[1] * ya.y + x[2] * ya.z;
-  const double tb = x[0] * yb.x + x[1] * yb.y + x[2] * yb.z;
-  double da = 1.0 - ta, db = 1.0 - tb;
+  double da = one_minus_dot<FUSE>(x, ya), db = one_minus_dot<FUSE>(x, yb);
   if (EPS) {
     da += eps2;
     db += eps2;
@@ -192,7 +201,8 @@ __device__ __forceinline__ void store_cross(const SegArgs& sg,
 // thread of the block must call it (it synchronises); `tile` is the
 // block's LDS staging array of TILE sources.  PAIR shares one reciprocal
 // between two sources.
-template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR>
+template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
+          bool FUSE = false>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
                                           int nsrc, const SegArgs& sg, int blk,
                                           double eps2, double4* tile) {
@@ -239,14 +249,15 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
         const double4 ya = tile[jj];
         const double4 yb = tile[jj + 1];
 #pragma unroll
-        for (int k = 0; k < TPL; ++k) f64_two<EPS, NR>(x[k], ya, yb, eps2, S[k]);
+        for (int k = 0; k < TPL; ++k)
+          f64_two<EPS, NR, FUSE>(x[k], ya, yb, eps2, S[k]);
       }
     } else {
 #pragma unroll 4
       for (int jj = 0; jj < TILE; ++jj) {
         const double4 y = tile[jj];
 #pragma unroll
-        for (int k = 0; k < TPL; ++k) f64_one<EPS, NR>(x[k], y, eps2, S[k]);
+        for (int k = 0; k < TPL; ++k) f64_one<EPS, NR, FUSE>(x[k], y, eps2, S[k]);
       }
     }
   }
@@ -595,6 +606,218 @@ __device__ __forceinline__ void mfma_block(const MfmaSrc& src, int nsrc,
     const double f = k == 0 ? Sf[0][a] : k == 1 ? Sf[1][a] : k == 2 ? Sf[2][a] : Sf[3][a];
     S[0][a] = f + Sn[a];
   }
+  store_cross<1>(sg, x, S, live, tgt);
+}
+
+// ---------------------------------------------- f16 hi/lo distance GEMM ---
+// Variant of the matrix-core far field whose distance product runs on the
+// f16 matrix cores (v_mfma_f32_32x32x16_f16: 32 sources x 32 targets per
+// instruction, f32 accumulation).  Per tile the offsets are scaled by
+// 1/S, S = max(R, kMfma16MinScale), so |u| = |y'|/S <= 1 and
+// |v| = |x'|/S <= 2/S, and every FP32 coordinate is split into
+// f16 hi + lo parts (22 significant bits).  With the K slots
+//   A_j = (uh, ul, uh, 1, 1, 1, qh, ql, 0, 0)      (q = |u|^2)
+//   B_i = (-2vh, -2vh, -2vl, ph, pl, pll, 1, 1, 0, 0)  (p = |v|^2 [+ eps])
+// (vectors count 3 slots, 16 in all) one instruction gives
+//   r^2 / S^2 = p + q - 2 (uh.vh + ul.vh + uh.vl),
+// missing only ul.vl (2^-22 |u||v|).  p, the dominant term of a far pair,
+// is split three ways so its own rounding stays near 2^-33 p.  The weight
+// product stays in FP32 (v_mfma_f32_4x4x1_16b_f32) against
+//   T_j = 2 s_j / S^2 (y'_j, 1)
+// stored for the 32x32 output layout: lane l, register v of the distance
+// product holds source 8 (v / 4) + 4 (l / 32) + v % 4, target l % 32.
+constexpr double kMfma16MinScale = 0.01;
+
+typedef _Float16 mh8 __attribute__((ext_vector_type(8)));
+typedef float mf16 __attribute__((ext_vector_type(16)));
+
+struct Mfma16Src {
+  const float4* y;   // (y', 1) FP32 for the near path
+  const float* s;    // 2 s
+  const uint4* a;    // two uint4 (16 f16) per source
+  const float* t;    // 1024 floats per tile
+  const double4* c;  // (c, R^2) per tile
+  const double* is;  // 1 / S per tile
+};
+
+__device__ __forceinline__ uint32_t pack_h2(_Float16 lo, _Float16 hi) {
+  return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, lo)) |
+         (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, hi)) << 16);
+}
+
+__device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo) {
+  hi = static_cast<_Float16>(x);
+  lo = static_cast<_Float16>(x - static_cast<float>(hi));
+}
+
+// One block per 256-source tile, after pack_mfma_kernel (reads its
+// centroids and radii): the f16 records, 1/S and the weight operand.
+__global__ __launch_bounds__(kPackTile) void pack_mfma16_kernel(
+    const double4* __restrict__ p64, const int32_t* __restrict__ count,
+    const double4* __restrict__ mf_c, uint4* __restrict__ mf_a,
+    float* __restrict__ mf_t32, double* __restrict__ mf_is) {
+  const int n = *count;
+  const int j0 = blockIdx.x * kPackTile;
+  if (j0 >= n) return;
+  const int j = j0 + threadIdx.x;
+  double4 y = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (j < n) y = p64[j];
+  const bool real = (j < n) && !(y.x == 0.0 && y.y == 0.0 && y.z == 0.0 &&
+                                 y.w == 0.0);
+  const double4 c = mf_c[blockIdx.x];
+  const double S = fmax(sqrt(c.w), kMfma16MinScale);
+  const double inv = 1.0 / S;
+  if (threadIdx.x == 0) mf_is[blockIdx.x] = inv;
+  const double e0 = y.x - c.x, e1 = y.y - c.y, e2 = y.z - c.z;
+  float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
+  double ws = 0.0;
+  if (real) {
+    u0 = static_cast<float>(e0 * inv);
+    u1 = static_cast<float>(e1 * inv);
+    u2 = static_cast<float>(e2 * inv);
+    ws = 2.0 * y.w * inv * inv;
+  }
+  const float q = u0 * u0 + u1 * u1 + u2 * u2;
+  _Float16 h0, l0, h1, l1, h2, l2, qh, ql;
+  split_h(u0, h0, l0);
+  split_h(u1, h1, l1);
+  split_h(u2, h2, l2);
+  split_h(q, qh, ql);
+  const _Float16 one = static_cast<_Float16>(1.0f), zero = static_cast<_Float16>(0.0f);
+  // K 0..7: uh0 uh1 uh2 ul0 ul1 ul2 uh0 uh1; K 8..15: uh2 1 1 1 qh ql 0 0
+  mf_a[2 * static_cast<int64_t>(j)] =
+      make_uint4(pack_h2(h0, h1), pack_h2(h2, l0), pack_h2(l1, l2), pack_h2(h0, h1));
+  mf_a[2 * static_cast<int64_t>(j) + 1] =
+      make_uint4(pack_h2(h2, one), pack_h2(one, one), pack_h2(qh, ql),
+                 pack_h2(zero, zero));
+  // weight operand: local source i = 32 ch + 8 (v / 4) + 4 h + v % 4
+  const int jj = threadIdx.x, ch = jj >> 5, i = jj & 31;
+  const int h = (i >> 2) & 1, v = 4 * (i >> 3) + (i & 3);
+  float* tt = mf_t32 + static_cast<int64_t>(blockIdx.x) * 1024 +
+              ((ch * 2 + h) * 4) * 16 + v;
+  tt[0] = static_cast<float>(ws * e0);
+  tt[16] = static_cast<float>(ws * e1);
+  tt[32] = static_cast<float>(ws * e2);
+  tt[48] = static_cast<float>(ws);
+}
+
+template <bool EPS>
+__device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
+                                             const SegArgs& sg, int blk,
+                                             float eps2x2, float4* ly,
+                                             float* ls, uint4* la, float4* lt) {
+  double x[1][3];
+  int self[1];
+  bool live[1];
+  int64_t tgt[1];
+  if (!load_targets<256, 1>(sg, blk, x, self, live, tgt)) return;
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5;    // K half of the operands, source half of D
+  const int col = lane & 31;  // target column
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+    const int t = j0 / kPackTile;
+    __syncthreads();
+    {
+      const int64_t base = static_cast<int64_t>(t) * kPackTile + threadIdx.x;
+      ly[threadIdx.x] = src.y[base];
+      ls[threadIdx.x] = src.s[base];
+      la[2 * threadIdx.x] = src.a[2 * base];
+      la[2 * threadIdx.x + 1] = src.a[2 * base + 1];
+      lt[threadIdx.x] = reinterpret_cast<const float4*>(src.t)[base];
+    }
+    __syncthreads();
+    const double4 c = src.c[t];
+    const double e0 = x[0][0] - c.x, e1 = x[0][1] - c.y, e2 = x[0][2] - c.z;
+    const bool near =
+        e0 * e0 + e1 * e1 + e2 * e2 <= kMfmaNear * kMfmaNear * c.w;
+    if (__ballot(near)) {
+      const float xr0 = static_cast<float>(e0), xr1 = static_cast<float>(e1),
+                  xr2 = static_cast<float>(e2);
+      const bool diag = static_cast<unsigned>(self[0] - j0) <
+                        static_cast<unsigned>(kPackTile);
+      float T0 = 0.0f, T1 = 0.0f, T2 = 0.0f;
+      for (int jj = 0; jj < kPackTile; ++jj) {
+        const float4 y = ly[jj];
+        const float d0 = y.x - xr0, d1 = y.y - xr1, d2 = y.z - xr2;
+        float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (EPS) r2 += eps2x2;
+        float w = ls[jj] * __builtin_amdgcn_rcpf(r2);
+        if (diag && j0 + jj == self[0]) w = 0.0f;
+        T0 = __builtin_fmaf(w, d0, T0);
+        T1 = __builtin_fmaf(w, d1, T1);
+        T2 = __builtin_fmaf(w, d2, T2);
+      }
+      S0 += static_cast<double>(T0);
+      S1 += static_cast<double>(T1);
+      S2 += static_cast<double>(T2);
+      continue;
+    }
+    // this lane's target in tile coordinates, split
+    const double inv = src.is[t];
+    const float v0 = static_cast<float>(e0 * inv), v1 = static_cast<float>(e1 * inv),
+                v2 = static_cast<float>(e2 * inv);
+    float p = v0 * v0 + v1 * v1 + v2 * v2;
+    if (EPS) p += eps2x2 * static_cast<float>(inv * inv);
+    _Float16 a0, b0, a1, b1, a2, b2, ph, pl, pll;
+    split_h(-2.0f * v0, a0, b0);
+    split_h(-2.0f * v1, a1, b1);
+    split_h(-2.0f * v2, a2, b2);
+    split_h(p, ph, pl);
+    pll = static_cast<_Float16>(p - static_cast<float>(ph) - static_cast<float>(pl));
+    const _Float16 one = static_cast<_Float16>(1.0f), zero = static_cast<_Float16>(0.0f);
+    // K 0..7: -2vh0 -2vh1 -2vh2 -2vh0 -2vh1 -2vh2 -2vl0 -2vl1
+    // K 8..15: -2vl2 ph pl pll 1 1 0 0
+    const uint32_t own[8] = {pack_h2(a0, a1), pack_h2(a2, a0), pack_h2(a1, a2),
+                             pack_h2(b0, b1), pack_h2(b2, ph), pack_h2(pl, pll),
+                             pack_h2(one, one), pack_h2(zero, zero)};
+    mh8 B[2];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) {
+      const int from = 32 * tb + col;
+      uint32_t g[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        g[e] = static_cast<uint32_t>(__shfl(static_cast<int>(own[e]), from, 64));
+      const uint4 sel = h ? make_uint4(g[4], g[5], g[6], g[7])
+                          : make_uint4(g[0], g[1], g[2], g[3]);
+      B[tb] = __builtin_bit_cast(mh8, sel);
+    }
+    mf4 D2[2] = {mf4{0.0f, 0.0f, 0.0f, 0.0f}, mf4{0.0f, 0.0f, 0.0f, 0.0f}};
+    const int c4 = lane & 3;
+#pragma unroll 1
+    for (int ch = 0; ch < kPackTile / 32; ++ch) {
+      const mh8 A = __builtin_bit_cast(mh8, la[(32 * ch + col) * 2 + h]);
+      const float4* tp = lt + ((ch * 2 + h) * 4 + c4) * 4;
+      const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+      const float tv[16] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w,
+                            t2.x, t2.y, t2.z, t2.w, t3.x, t3.y, t3.z, t3.w};
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) {
+        const mf16 D = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+            A, B[tb], mf16{}, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+          D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+              tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb], 0, 0, 0);
+      }
+    }
+    // the two source halves (lanes l, l ^ 32) of each column; lane l keeps
+    // column block h, which holds its own target 32 h + col = l
+    float o[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      float v0s = D2[0][a], v1s = D2[1][a];
+      v0s += __shfl_xor(v0s, 32, 64);
+      v1s += __shfl_xor(v1s, 32, 64);
+      o[a] = h ? v1s : v0s;
+    }
+    const double W = static_cast<double>(o[3]);
+    S0 += static_cast<double>(o[0]) + c.x * W;
+    S1 += static_cast<double>(o[1]) + c.y * W;
+    S2 += static_cast<double>(o[2]) + c.z * W;
+  }
+  double S[1][3] = {{S0, S1, S2}};
   store_cross<1>(sg, x, S, live, tgt);
 }
 

@@ -103,7 +103,7 @@ __device__ __forceinline__ int ens_next(const EnsArgs& A, int* slot) {
   return it;
 }
 
-template <int BLOCK, int TPL, int NR, bool PAIR>
+template <int BLOCK, int TPL, int NR, bool PAIR, bool FUSE = false>
 __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
     const double4* __restrict__ src, EnsArgs A) {
   constexpr int TILE = 256;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
     const int4 w = A.items[it];
     const EnsMember M = A.mem[w.x];
     const SegArgs sg = ens_segment(A, M, w.y);
-    f64_block<BLOCK, TPL, TILE, false, NR, PAIR>(src + M.P, M.cnt, sg, w.z,
+    f64_block<BLOCK, TPL, TILE, false, NR, PAIR, FUSE>(src + M.P, M.cnt, sg, w.z,
                                                  0.0, tile);
   }
 }
@@ -157,11 +157,33 @@ __global__ __launch_bounds__(256) void ens_mfma_kernel(MfmaSrc src, EnsArgs A) {
   }
 }
 
+__global__ __launch_bounds__(256) void ens_mfma16_kernel(Mfma16Src src,
+                                                         EnsArgs A) {
+  __shared__ float4 ly[kPackTile];
+  __shared__ float ls[kPackTile];
+  __shared__ uint4 la[2 * kPackTile];
+  __shared__ float4 lt[kPackTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = ens_next(A, &slot);
+    if (it >= A.n_items) break;
+    const int4 w = A.items[it];
+    const EnsMember M = A.mem[w.x];
+    const SegArgs sg = ens_segment(A, M, w.y);
+    const int t0 = M.P / kPackTile;
+    Mfma16Src ms{src.y + M.P, src.s + M.P, src.a + 2 * static_cast<int64_t>(M.P),
+                 src.t + static_cast<int64_t>(t0) * 1024, src.c + t0,
+                 src.is + t0};
+    mfma16_block<false>(ms, M.cnt, sg, w.z, 0.0f, ly, ls, la, lt);
+  }
+}
+
 // ------------------------------------------------------ kernel registry ---
 // Every variant has a one-problem twin of the same shape in bve_sums.hip.
 using EnsF64Fn = void (*)(const double4*, EnsArgs);
 using EnsMixedFn = void (*)(const float4*, const double4*, EnsArgs);
 using EnsMfmaFn = void (*)(MfmaSrc, EnsArgs);
+using EnsMfma16Fn = void (*)(Mfma16Src, EnsArgs);
 
 struct EnsVariant {
   EnsF64Fn f64;
@@ -169,10 +191,12 @@ struct EnsVariant {
   int block, tpl;
   const char* name;
   EnsMfmaFn mfma;
+  EnsMfma16Fn mfma16;
   const void* fn() const {
     return f64 ? reinterpret_cast<const void*>(f64)
-               : mixed ? reinterpret_cast<const void*>(mixed)
-                       : reinterpret_cast<const void*>(mfma);
+           : mixed ? reinterpret_cast<const void*>(mixed)
+           : mfma ? reinterpret_cast<const void*>(mfma)
+                  : reinterpret_cast<const void*>(mfma16);
   }
 };
 
@@ -189,6 +213,10 @@ const EnsVariant kEnsF64[] = {
     LPM_ENS_F64(256, 1, 1, true),  // 2
     LPM_ENS_F64(256, 4, 1, true),  // 3
     LPM_ENS_F64(128, 2, 2, true),  // 4
+    {ens_f64_kernel<256, 2, 1, true, true>, nullptr, 256, 2,
+     "persistent_block256_tpl2_nr1_pairtrue_fused", nullptr},  // 5
+    {ens_f64_kernel<256, 4, 1, true, true>, nullptr, 256, 4,
+     "persistent_block256_tpl4_nr1_pairtrue_fused", nullptr},  // 6
 };
 const EnsVariant kEnsMixed[] = {
     LPM_ENS_MIXED(256, 2, false),  // 0: fixed default
@@ -199,6 +227,8 @@ const EnsVariant kEnsMixed[] = {
     LPM_ENS_MIXED(128, 2, true),   // 5
     {nullptr, nullptr, 256, 1, "persistent_mfma_block256_near3",
      ens_mfma_kernel},             // 6: matrix-core far field
+    {nullptr, nullptr, 256, 1, "persistent_mfma16_block256_near3", nullptr,
+     ens_mfma16_kernel},           // 7: f16-split distance product
 };
 constexpr int kNumEnsF64 = sizeof(kEnsF64) / sizeof(kEnsF64[0]);
 constexpr int kNumEnsMixed = sizeof(kEnsMixed) / sizeof(kEnsMixed[0]);
@@ -242,6 +272,9 @@ void ensure_sources(lpm_ctx* ctx, const EnsembleDesc& d, int precision) {
     ctx->mf_s.ensure(sizeof(float) * cap);
     ctx->mf_t.ensure(sizeof(float) * 4 * cap);
     ctx->mf_c.ensure(sizeof(double4) * (cap / kPackTile));
+    ctx->mf_a.ensure(sizeof(uint4) * 2 * cap);
+    ctx->mf_t32.ensure(sizeof(float) * 4 * cap);
+    ctx->mf_is.ensure(sizeof(double) * (cap / kPackTile));
   }
 }
 
@@ -297,11 +330,12 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
   const float4* p32 = nullptr;
   const double4* cen = nullptr;
   MfmaSrc ms{};
+  Mfma16Src ms16{};
   if (precision == LPM_MIXED) {
     const int ntile = static_cast<int>(d.total_padded_upper / kPackTile);
     p32 = ctx->packed32.as<float4>();
     cen = ctx->centers.as<double4>();
-    if (v.mfma) {
+    if (v.mfma || v.mfma16) {
       ms = MfmaSrc{ctx->mf_y.as<float4>(), ctx->mf_q.as<float>(),
                    ctx->mf_s.as<float>(), ctx->mf_t.as<float>(),
                    ctx->mf_c.as<double4>()};
@@ -310,6 +344,15 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
           p64, L.padded_total, ctx->mf_y.as<float4>(), ctx->mf_q.as<float>(),
           ctx->mf_s.as<float>(), ctx->mf_t.as<float>(),
           ctx->mf_c.as<double4>());
+      if (v.mfma16) {
+        LPM_HIP_CHECK(hipGetLastError());
+        pack_mfma16_kernel<<<ntile, kPackTile, 0, s>>>(
+            p64, L.padded_total, ctx->mf_c.as<double4>(), ctx->mf_a.as<uint4>(),
+            ctx->mf_t32.as<float>(), ctx->mf_is.as<double>());
+        ms16 = Mfma16Src{ctx->mf_y.as<float4>(), ctx->mf_s.as<float>(),
+                         ctx->mf_a.as<uint4>(), ctx->mf_t32.as<float>(),
+                         ctx->mf_c.as<double4>(), ctx->mf_is.as<double>()};
+      }
     } else {
       pack_mixed_kernel<<<ntile, kPackTile, 0, s>>>(
           p64, L.padded_total, ctx->packed32.as<float4>(),
@@ -320,7 +363,9 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
   ensemble_reset_queue(L, s);
   const int grid = ensemble_grid(ctx, v.fn(), v.block, A.n_items);
   TimedLaunch tl(ctx, family, s, L.pairs);
-  if (v.mfma)
+  if (v.mfma16)
+    hipLaunchKernelGGL(v.mfma16, dim3(grid), dim3(v.block), 0, s, ms16, A);
+  else if (v.mfma)
     hipLaunchKernelGGL(v.mfma, dim3(grid), dim3(v.block), 0, s, ms, A);
   else if (precision == LPM_MIXED)
     hipLaunchKernelGGL(v.mixed, dim3(grid), dim3(v.block), 0, s, p32, cen, A);

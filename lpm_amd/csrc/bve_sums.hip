@@ -133,7 +133,8 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(
 // ------------------------------------------------------- velocity sums ---
 // One launch covers two target segments (vertices, then faces): blocks
 // [0, s0.nblocks) take s0, the rest s1.
-template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR>
+template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
+          bool FUSE = false>
 __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
     SegArgs s0, SegArgs s1, double eps2) {
@@ -141,7 +142,8 @@ __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR>(src, *count, sg, blk, eps2, tile);
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE>(src, *count, sg, blk, eps2,
+                                                  tile);
 }
 
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
@@ -168,6 +170,20 @@ __global__ __launch_bounds__(256) void bve_vel_mfma_kernel(
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
   mfma_block<EPS>(src, *count, sg, blk, eps2x2, ly, lq, ls, lt);
+}
+
+template <bool EPS>
+__global__ __launch_bounds__(256) void bve_vel_mfma16_kernel(
+    Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
+    float eps2x2) {
+  __shared__ float4 ly[kPackTile];
+  __shared__ float ls[kPackTile];
+  __shared__ uint4 la[2 * kPackTile];
+  __shared__ float4 lt[kPackTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  mfma16_block<EPS>(src, *count, sg, blk, eps2x2, ly, ls, la, lt);
 }
 
 // ------------------------------------------------------ stream function ---
@@ -235,6 +251,8 @@ using F64Kernel = void (*)(const double4*, const int32_t*, SegArgs, SegArgs,
 using MixedKernel = void (*)(const float4*, const double4*, const int32_t*,
                              SegArgs, SegArgs, float);
 using MfmaKernel = void (*)(MfmaSrc, const int32_t*, SegArgs, SegArgs, float);
+using Mfma16Kernel = void (*)(Mfma16Src, const int32_t*, SegArgs, SegArgs,
+                              float);
 
 struct F64Variant {
   F64Kernel fn, fn_eps;
@@ -246,12 +264,17 @@ struct MixedVariant {
   MfmaKernel mfn, mfn_eps; // matrix-core kernels (or null)
   int block, tpl;
   const char* name;
+  Mfma16Kernel m16, m16_eps;  // f16-split matrix-core kernels (or null)
 };
 
 #define LPM_F64_VARIANT(B, T, NR, PAIR)                                  \
   {bve_vel_f64_kernel<B, T, 256, false, NR, PAIR>,                      \
    bve_vel_f64_kernel<B, T, 256, true, NR, PAIR>, B, T,                 \
    "block" #B "_tpl" #T "_nr" #NR "_pair" #PAIR}
+#define LPM_F64_FUSED(B, T, NR)                                          \
+  {bve_vel_f64_kernel<B, T, 256, false, NR, true, true>,                \
+   bve_vel_f64_kernel<B, T, 256, true, NR, true, true>, B, T,           \
+   "block" #B "_tpl" #T "_nr" #NR "_pairtrue_fused"}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -266,6 +289,9 @@ const F64Variant kF64Variants[] = {
     LPM_F64_VARIANT(256, 2, 1, true),   // 5: one Newton step
     LPM_F64_VARIANT(256, 4, 1, true),   // 6
     LPM_F64_VARIANT(256, 1, 1, true),   // 7
+    LPM_F64_FUSED(256, 2, 1),           // 8: 1 - x.y as one FMA chain
+    LPM_F64_FUSED(256, 4, 1),           // 9
+    LPM_F64_FUSED(256, 1, 1),           // 10
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
@@ -277,6 +303,8 @@ const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 4, false),  // 6
     {nullptr, nullptr, bve_vel_mfma_kernel<false>, bve_vel_mfma_kernel<true>,
      256, 1, "mfma_block256_near3"},   // 7: matrix-core far field
+    {nullptr, nullptr, nullptr, nullptr, 256, 1, "mfma16_block256_near3",
+     bve_vel_mfma16_kernel<false>, bve_vel_mfma16_kernel<true>},  // 8: f16 split
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
@@ -436,6 +464,15 @@ PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
     ps.mf_s = m.s;
     ps.mf_t = m.t;
     ps.mf_c = m.c;
+    uint4* a16 = static_cast<uint4*>(ctx->mf_a.ensure(sizeof(uint4) * 2 * tcap));
+    float* t32 = static_cast<float*>(ctx->mf_t32.ensure(sizeof(float) * 4 * tcap));
+    double* is = static_cast<double*>(ctx->mf_is.ensure(sizeof(double) * ntile));
+    pack_mfma16_kernel<<<ntile, kPackTile, 0, s>>>(p64, map.d_count, m.c, a16,
+                                                   t32, is);
+    LPM_HIP_CHECK(hipGetLastError());
+    ps.mf_a = a16;
+    ps.mf_t32 = t32;
+    ps.mf_is = is;
   }
   return ps;
 }
@@ -470,7 +507,13 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
     TimedLaunch tl(ctx, KID_BVE_VEL_MIXED, s, pairs);
-    if (v.mfn) {
+    if (v.m16) {
+      LPM_ARG_CHECK(src.mf_a, "matrix-core layout not packed");
+      Mfma16Src m{src.mf_y, src.mf_s, src.mf_a, src.mf_t32, src.mf_c, src.mf_is};
+      hipLaunchKernelGGL(eps2 > 0 ? v.m16_eps : v.m16, dim3(grid),
+                         dim3(v.block), 0, s, m, src.count, a0, a1,
+                         static_cast<float>(2.0 * eps2));
+    } else if (v.mfn) {
       LPM_ARG_CHECK(src.mf_y, "matrix-core layout not packed");
       MfmaSrc m{src.mf_y, src.mf_q, src.mf_s, src.mf_t, src.mf_c};
       hipLaunchKernelGGL(eps2 > 0 ? v.mfn_eps : v.mfn, dim3(grid),

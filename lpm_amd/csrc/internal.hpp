@@ -109,6 +109,10 @@ struct PackedSources {
   const float* mf_s = nullptr;
   const float* mf_t = nullptr;
   const double4* mf_c = nullptr;
+  // mixed: f16 hi/lo distance operands and the 32x32 weight operand
+  const uint4* mf_a = nullptr;
+  const float* mf_t32 = nullptr;
+  const double* mf_is = nullptr;
 };
 
 // Kernel timing records (HIP events around direct-sum launches).
@@ -152,6 +156,7 @@ struct lpm_ctx {
   lpm_amd::DevBuf packed32;     // float4 per source (mixed)
   lpm_amd::DevBuf centers;      // double4 per tile (mixed)
   lpm_amd::DevBuf mf_y, mf_q, mf_s, mf_t, mf_c;  // matrix-core layout (mixed)
+  lpm_amd::DevBuf mf_a, mf_t32, mf_is;          // f16-split layout (mixed)
   lpm_amd::DevBuf flag;         // int32 non-finite flag
   lpm_amd::DevBuf count_ident;  // int32 count when no mask
 
