@@ -617,6 +617,15 @@ def main(argv=None):
     ue_f = G.rh54_velocity(mesh.face_x[:, mesh.leaf_slice], 0.1, -0.1)
     an_v = errors(r64["u0_v"], ue_v)
     an_f = errors(r64["u0_f"], ue_f)
+    # the RH 5-4 wave rotates rigidly at nu (exact nonlinear solution): the
+    # velocity at the particles after the run vs the rotated pattern
+    t_end = (a.warmup + a.steps) * a.dt
+    nu = G.rh54_wave_speed(0.1, 2 * np.pi)
+
+    def rh54_at(x, t):
+        return G.rotate_z(G.rh54_velocity(G.rotate_z(x, -nu * t), 0.1, -0.1),
+                          nu * t)
+    an_t = errors(r64["uT_v"], rh54_at(r64["xT_v"], t_end))
 
     result = {
         "metric": METRIC,
@@ -644,6 +653,9 @@ def main(argv=None):
         "errors_fp64": {
             "vs_analytic_rh54_vertices_rel_l2_linf": an_v,
             "vs_analytic_rh54_faces_rel_l2_linf": an_f,
+            "vs_analytic_rh54_after_steps_vertices_rel_l2_linf": an_t,
+            "analytic_after_steps": f"t = {t_end:g}, pattern rotated by "
+                                    f"nu t = {nu * t_end:.5f} rad",
         },
         "roofline": {
             # compute-bound at the FP64 peak (78.6 TFLOP/s on MI355X for both
@@ -689,6 +701,8 @@ def main(argv=None):
             "velocity_error_vs_fp64_after_steps_vertices_rel_l2_linf":
                 errors(rmx["uT_v"], r64["uT_v"]),
             "vs_analytic_rh54_vertices_rel_l2_linf": errors(rmx["u0_v"], ue_v),
+            "vs_analytic_rh54_after_steps_vertices_rel_l2_linf":
+                errors(rmx["uT_v"], rh54_at(rmx["xT_v"], t_end)),
             "autotune": rmx["autotune"],
         }
         log(f"mixed: {mstep*1e3:.1f} ms/step, {mval:.1f} Gpairs/s "

@@ -27,6 +27,13 @@ def fixtures():
 
 
 @pytest.fixture(scope="session")
+def oracle_fixtures():
+    """Frozen oracle outputs (tests/golden/make_oracle_fixtures.py)."""
+    with np.load(os.path.join(GOLDEN, "oracle_fixtures.npz")) as f:
+        return {k: f[k] for k in f.files}
+
+
+@pytest.fixture(scope="session")
 def ora():
     import oracle
     oracle.lib()

@@ -287,6 +287,63 @@ def test_rk4_solid_body_rotation(ctx):
     assert err < 1e-2 * ang
 
 
+@pytest.mark.parametrize("seed,tag", [(0, "icos"), (1, "cube")])
+def test_frozen_oracle_fixture(ctx, oracle_fixtures, seed, tag):
+    # level 2, RH 5-4: velocity, stream function and one classical RK4 step
+    # against the committed oracle outputs (tests/golden)
+    from lpm_amd.bve import BVERK4
+    F = {k[len(tag) + 1:]: v for k, v in oracle_fixtures.items()
+         if k.startswith(tag + "_")}
+    s = _sphere(ctx, seed, 2)
+    s.init_velocity()
+    s.init_stream_fn()
+    torch.cuda.synchronize()
+    leaf = s.mesh.face_mask == 0
+    assert rel_l2(aos(s.velocity_verts.cpu().numpy()), F["vert_u"]) < TOL64
+    assert rel_l2(aos(s.velocity_faces.cpu().numpy())[leaf],
+                  F["face_u"][leaf]) < TOL64
+    assert rel_l2(s.stream_fn_verts.cpu().numpy(), F["vert_psi"]) < TOL64
+    solver = BVERK4(0.01, s, reference_compat=False)
+    solver.advance_timestep(s)
+    torch.cuda.synchronize()
+    assert rel_l2(aos(s.vert_x.cpu().numpy()), F["step_vert_x"]) < TOL64
+    assert rel_l2(aos(s.face_x.cpu().numpy())[leaf], F["step_face_x"][leaf]) < TOL64
+    assert rel_l2(s.rel_vort_verts.cpu().numpy(), F["step_vert_zeta"]) < TOL64
+    assert rel_l2(s.rel_vort_faces.cpu().numpy()[leaf],
+                  F["step_face_zeta"][leaf]) < TOL64
+    assert rel_l2(aos(s.velocity_verts.cpu().numpy()), F["step_vert_u"]) < 1e-11
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_rh54_wave_moves_at_its_phase_speed(ctx, prec):
+    # RH 5-4 is an exact solution of the nonlinear BVE that rotates rigidly
+    # in longitude at nu = -0.3255 (u0 = 0.1, Omega = 2 pi): after 20 RK4
+    # steps the velocity at the particles matches the rotated pattern to
+    # the t = 0 quadrature error and is far from the unrotated pattern
+    from lpm_amd import galleries as G
+    from lpm_amd.bve import BVERK4
+    s = _sphere(ctx, 0, 5)
+    s.init_velocity(precision=prec, flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    nu = G.rh54_wave_speed(0.1, OMEGA)
+
+    def errs(t):
+        x = s.vert_x.cpu().numpy()
+        u = s.velocity_verts.cpu().numpy()
+        ue = G.rotate_z(G.rh54_velocity(G.rotate_z(x, -nu * t), 0.1, -0.1), nu * t)
+        us = G.rh54_velocity(x, 0.1, -0.1)
+        return rel_l2(u, ue), rel_l2(u, us)
+
+    e0, _ = errs(0.0)
+    solver = BVERK4(0.01, s, precision=prec, reference_compat=False,
+                    leaf_only=True, renormalize=True)
+    for _ in range(20):
+        solver.advance_timestep(s)
+    torch.cuda.synchronize()
+    e_rot, e_static = errs(0.2)
+    assert e_rot < 1.6 * e0, (e0, e_rot)
+    assert e_static > 15 * e_rot, (e_rot, e_static)
+
+
 @pytest.mark.parametrize("kid,prec", [(_abi.LPM_KERNEL_BVE_VEL_F64, "fp64"),
                                       (_abi.LPM_KERNEL_BVE_VEL_MIXED, "mixed")])
 def test_every_variant_matches_oracle(ctx, ora, kid, prec):

@@ -10,6 +10,8 @@ dropped term) fails at least one:
   * RK4 order and exact solid-body rotation of positions;
   * closed-form Laplacians for the PSE operator, TC2 closed forms for SWE.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -263,6 +265,32 @@ def test_rk4_solid_body_rotation_positions(ora):
     # a wrong sign or a dropped stage moves the particles the wrong way
     wrong = np.max(np.linalg.norm(x0 - ex, axis=1))
     assert err < 0.1 * wrong
+
+
+def _rh54_errors(u, x, t, u0=0.1, amp=-0.1):
+    """Relative L2 of velocities u (n, 3) at positions x against the
+    Rossby-Haurwitz 5-4 wave at time t (the t = 0 pattern rotated in
+    longitude by nu t) and against the unrotated pattern."""
+    from lpm_amd import galleries as G
+    nu = G.rh54_wave_speed(u0, OMEGA)
+    ue = G.rotate_z(G.rh54_velocity(G.rotate_z(x.T, -nu * t), u0, amp), nu * t)
+    us = G.rh54_velocity(x.T, u0, amp)
+    return rel_l2(u, ue.T), rel_l2(u, us.T)
+
+
+def test_rk4_rh54_wave_moves_at_its_phase_speed(ora):
+    # the RH 5-4 wave is an exact solution of the nonlinear BVE that rotates
+    # rigidly at nu = (R(3+R) u0 - 2 Omega)/((1+R)(2+R)) = -0.3255: after 20
+    # steps the velocity matches the rotated pattern to the quadrature error
+    # of t = 0, and is far from the unrotated one
+    m = mesh(0, 3)
+    s = _rk_state(m, lambda x: ora.rh54_vorticity(x, 0.1, -0.1), ora)
+    e0, _ = _rh54_errors(s["vu"], s["vx"], 0.0)
+    for _ in range(20):
+        ora.bve_rk4_step(s, 0.01, OMEGA, compat=False, renorm=True)
+    e_rot, e_static = _rh54_errors(s["vu"], s["vx"], 0.2)
+    assert e_rot < 1.6 * e0, (e0, e_rot)
+    assert e_static > 15 * e_rot, (e_rot, e_static)
 
 
 def test_rk4_order(ora):
@@ -585,3 +613,23 @@ def test_ic2d_rk2_second_order(ora):
         ends.append(s["vx"].copy())
     r = np.abs(ends[0] - ends[1]).max() / np.abs(ends[1] - ends[2]).max()
     assert 3.5 < r < 4.5
+
+
+# ------------------------------------------------- frozen oracle outputs
+@pytest.mark.parametrize("seed,tag", [(0, "icos"), (1, "cube")])
+def test_oracle_reproduces_its_frozen_outputs(ora, oracle_fixtures, seed, tag):
+    # guards the restatement itself against regressions: the committed
+    # fixture was written by tests/golden/make_oracle_fixtures.py
+    import importlib.util
+    path = os.path.join(os.path.dirname(__file__), "golden",
+                        "make_oracle_fixtures.py")
+    spec = importlib.util.spec_from_file_location("mk_oracle_fixtures", path)
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    got = mk.fixture_arrays(ora, seed)
+    leaf = mesh(seed, 2).face_mask == 0
+    for k, v in got.items():
+        ref = oracle_fixtures[f"{tag}_{k}"]
+        if "face" in k:  # divided icosahedral faces carry 1/0 velocities
+            v, ref = v[leaf], ref[leaf]
+        assert rel_l2(v, ref) < 1e-15, k
