@@ -108,6 +108,42 @@ __device__ __forceinline__ void f64_two(const double* x, const double4& ya,
   S[2] = __builtin_fma(wb, yb.z, S[2]);
 }
 
+// Four sources sharing one reciprocal: r = 1/(da db dc dd),
+// 1/da = db (r dc dd), 1/db = da (r dc dd), and so on; terms in source order.
+template <bool EPS, int NR, bool FUSE>
+__device__ __forceinline__ void f64_four(const double* x, const double4& ya,
+                                         const double4& yb, const double4& yc,
+                                         const double4& yd, double eps2,
+                                         double* S) {
+  double da = one_minus_dot<FUSE>(x, ya), db = one_minus_dot<FUSE>(x, yb);
+  double dc = one_minus_dot<FUSE>(x, yc), dd = one_minus_dot<FUSE>(x, yd);
+  if (EPS) {
+    da += eps2;
+    db += eps2;
+    dc += eps2;
+    dd += eps2;
+  }
+  const double pab = da * db, pcd = dc * dd;
+  const double r = rcp_nr<NR>(pab * pcd);
+  const double rab = r * pcd, rcd = r * pab;
+  const double wa = ya.w * (db * rab);
+  const double wb = yb.w * (da * rab);
+  const double wc = yc.w * (dd * rcd);
+  const double wd = yd.w * (dc * rcd);
+  S[0] = __builtin_fma(wa, ya.x, S[0]);
+  S[1] = __builtin_fma(wa, ya.y, S[1]);
+  S[2] = __builtin_fma(wa, ya.z, S[2]);
+  S[0] = __builtin_fma(wb, yb.x, S[0]);
+  S[1] = __builtin_fma(wb, yb.y, S[1]);
+  S[2] = __builtin_fma(wb, yb.z, S[2]);
+  S[0] = __builtin_fma(wc, yc.x, S[0]);
+  S[1] = __builtin_fma(wc, yc.y, S[1]);
+  S[2] = __builtin_fma(wc, yc.z, S[2]);
+  S[0] = __builtin_fma(wd, yd.x, S[0]);
+  S[1] = __builtin_fma(wd, yd.y, S[1]);
+  S[2] = __builtin_fma(wd, yd.z, S[2]);
+}
+
 // Mixed: d = y' - x' (tile-relative FP32), r2 = |d|^2 + 2 eps^2,
 // T += (2 s / r2) d.
 template <bool EPS>
@@ -202,7 +238,7 @@ __device__ __forceinline__ void store_cross(const SegArgs& sg,
 // block's LDS staging array of TILE sources.  PAIR shares one reciprocal
 // between two sources.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false>
+          bool FUSE = false, bool QUAD = false>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
                                           int nsrc, const SegArgs& sg, int blk,
                                           double eps2, double4* tile) {
@@ -242,6 +278,14 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
           S[k][1] = __builtin_fma(w, y.y, S[k][1]);
           S[k][2] = __builtin_fma(w, y.z, S[k][2]);
         }
+      }
+    } else if (QUAD) {
+      for (int jj = 0; jj < TILE; jj += 4) {
+        const double4 ya = tile[jj], yb = tile[jj + 1];
+        const double4 yc = tile[jj + 2], yd = tile[jj + 3];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k)
+          f64_four<EPS, NR, FUSE>(x[k], ya, yb, yc, yd, eps2, S[k]);
       }
     } else if (PAIR) {
 #pragma unroll 2

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(
 // One launch covers two target segments (vertices, then faces): blocks
 // [0, s0.nblocks) take s0, the rest s1.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false>
+          bool FUSE = false, bool QUAD = false>
 __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
     SegArgs s0, SegArgs s1, double eps2) {
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE>(src, *count, sg, blk, eps2,
-                                                  tile);
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD>(src, *count, sg, blk,
+                                                        eps2, tile);
 }
 
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
@@ -275,6 +275,10 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, 256, false, NR, true, true>,                \
    bve_vel_f64_kernel<B, T, 256, true, NR, true, true>, B, T,           \
    "block" #B "_tpl" #T "_nr" #NR "_pairtrue_fused"}
+#define LPM_F64_QUAD(B, T, NR)                                           \
+  {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true>,          \
+   bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true>, B, T,     \
+   "block" #B "_tpl" #T "_nr" #NR "_quad_fused"}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -292,6 +296,8 @@ const F64Variant kF64Variants[] = {
     LPM_F64_FUSED(256, 2, 1),           // 8: 1 - x.y as one FMA chain
     LPM_F64_FUSED(256, 4, 1),           // 9
     LPM_F64_FUSED(256, 1, 1),           // 10
+    LPM_F64_QUAD(256, 2, 1),            // 11: one reciprocal per 4 pairs
+    LPM_F64_QUAD(256, 4, 1),            // 12
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
