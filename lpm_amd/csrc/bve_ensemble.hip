@@ -103,7 +103,8 @@ __device__ __forceinline__ int ens_next(const EnsArgs& A, int* slot) {
   return it;
 }
 
-template <int BLOCK, int TPL, int NR, bool PAIR, bool FUSE = false>
+template <int BLOCK, int TPL, int NR, bool PAIR, bool FUSE = false,
+          bool QUAD = false>
 __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
     const double4* __restrict__ src, EnsArgs A) {
   constexpr int TILE = 256;
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
     const int4 w = A.items[it];
     const EnsMember M = A.mem[w.x];
     const SegArgs sg = ens_segment(A, M, w.y);
-    f64_block<BLOCK, TPL, TILE, false, NR, PAIR, FUSE>(src + M.P, M.cnt, sg, w.z,
+    f64_block<BLOCK, TPL, TILE, false, NR, PAIR, FUSE, QUAD>(src + M.P, M.cnt, sg, w.z,
                                                  0.0, tile);
   }
 }
@@ -157,8 +158,9 @@ __global__ __launch_bounds__(256) void ens_mfma_kernel(MfmaSrc src, EnsArgs A) {
   }
 }
 
-__global__ __launch_bounds__(256) void ens_mfma16_kernel(Mfma16Src src,
-                                                         EnsArgs A) {
+template <int MINW>
+__global__ __launch_bounds__(256, MINW) void ens_mfma16_kernel(Mfma16Src src,
+                                                               EnsArgs A) {
   __shared__ float4 ly[kPackTile];
   __shared__ float ls[kPackTile];
   __shared__ uint4 la[2 * kPackTile];
@@ -217,6 +219,8 @@ const EnsVariant kEnsF64[] = {
      "persistent_block256_tpl2_nr1_pairtrue_fused", nullptr},  // 5
     {ens_f64_kernel<256, 4, 1, true, true>, nullptr, 256, 4,
      "persistent_block256_tpl4_nr1_pairtrue_fused", nullptr},  // 6
+    {ens_f64_kernel<256, 4, 1, true, true, true>, nullptr, 256, 4,
+     "persistent_block256_tpl4_nr1_quad_fused", nullptr},      // 7
 };
 const EnsVariant kEnsMixed[] = {
     LPM_ENS_MIXED(256, 2, false),  // 0: fixed default
@@ -228,7 +232,9 @@ const EnsVariant kEnsMixed[] = {
     {nullptr, nullptr, 256, 1, "persistent_mfma_block256_near3",
      ens_mfma_kernel},             // 6: matrix-core far field
     {nullptr, nullptr, 256, 1, "persistent_mfma16_block256_near3", nullptr,
-     ens_mfma16_kernel},           // 7: f16-split distance product
+     ens_mfma16_kernel<1>},        // 7: f16-split distance product
+    {nullptr, nullptr, 256, 1, "persistent_mfma16_occ5_block256_near3", nullptr,
+     ens_mfma16_kernel<5>},        // 8: register budget for 5 waves per SIMD
 };
 constexpr int kNumEnsF64 = sizeof(kEnsF64) / sizeof(kEnsF64[0]);
 constexpr int kNumEnsMixed = sizeof(kEnsMixed) / sizeof(kEnsMixed[0]);

@@ -172,8 +172,9 @@ __global__ __launch_bounds__(256) void bve_vel_mfma_kernel(
   mfma_block<EPS>(src, *count, sg, blk, eps2x2, ly, lq, ls, lt);
 }
 
-template <bool EPS>
-__global__ __launch_bounds__(256) void bve_vel_mfma16_kernel(
+// MINW: minimum waves per SIMD asked of the register allocator (1: none).
+template <bool EPS, int MINW = 1>
+__global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
     Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
     float eps2x2) {
   __shared__ float4 ly[kPackTile];
@@ -311,6 +312,8 @@ const MixedVariant kMixedVariants[] = {
      256, 1, "mfma_block256_near3"},   // 7: matrix-core far field
     {nullptr, nullptr, nullptr, nullptr, 256, 1, "mfma16_block256_near3",
      bve_vel_mfma16_kernel<false>, bve_vel_mfma16_kernel<true>},  // 8: f16 split
+    {nullptr, nullptr, nullptr, nullptr, 256, 1, "mfma16_occ5_block256_near3",
+     bve_vel_mfma16_kernel<false, 5>, bve_vel_mfma16_kernel<true, 5>},  // 9
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);

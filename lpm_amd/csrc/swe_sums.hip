@@ -439,6 +439,22 @@ __device__ __forceinline__ void swe_mixed_block(
       if (diag && j0 + jj == self) r = 0.0f;
       swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
     }
+    // carry the tile's FP32 partials into the FP64 accumulators
+    acc.A0 += p.A0;
+    acc.Bs += p.Bs;
+    acc.Sbd += p.Sbd;
+    acc.Sd2 += p.Sd2;
+
+    for (int k = 0; k < 3; ++k) {
+      acc.Vd[k] += p.Vd[k];
+      acc.Wd[k] += p.Wd[k];
+      acc.Vs[k] += p.Vs[k];
+      acc.Ws[k] += p.Ws[k];
+    }
+    for (int k = 0; k < 6; ++k) {
+      acc.Yd[k] += p.Yd[k];
+      acc.Ys[k] += p.Ys[k];
+    }
     if (do_pse) {
       const double e0 = x[0] - bd.x, e1 = x[1] - bd.y, e2 = x[2] - bd.z;
       const double dist2 = e0 * e0 + e1 * e1 + e2 * e2;
@@ -460,22 +476,6 @@ __device__ __forceinline__ void swe_mixed_block(
         acc.L1 += L;
       }
     }
-    // carry the tile's FP32 partials into the FP64 accumulators
-    acc.A0 += p.A0;
-    acc.Bs += p.Bs;
-    acc.Sbd += p.Sbd;
-    acc.Sd2 += p.Sd2;
-
-    for (int k = 0; k < 3; ++k) {
-      acc.Vd[k] += p.Vd[k];
-      acc.Wd[k] += p.Wd[k];
-      acc.Vs[k] += p.Vs[k];
-      acc.Ws[k] += p.Ws[k];
-    }
-    for (int k = 0; k < 6; ++k) {
-      acc.Yd[k] += p.Yd[k];
-      acc.Ys[k] += p.Ys[k];
-    }
   }
   if (!in) return;
   double u[3], dd, lap;
@@ -492,7 +492,10 @@ __device__ __forceinline__ void swe_mixed_block(
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_mixed_kernel(
+// at least 3 waves per SIMD: the FP64 accumulators, the FP32 tile partials
+// and the PSE loop otherwise hold 198 registers (2 waves); the few spills
+// this costs sit outside the pair loop
+__global__ __launch_bounds__(BLOCK, 3) void swe_mixed_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
   __shared__ F4x2 tile[kSweTile];
@@ -622,7 +625,10 @@ __global__ __launch_bounds__(BLOCK) void swe_ens_f64_kernel(
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_ens_mixed_kernel(
+// at least 3 waves per SIMD: the FP64 accumulators, the FP32 tile partials
+// and the PSE loop otherwise hold 198 registers (2 waves); the few spills
+// this costs sit outside the pair loop
+__global__ __launch_bounds__(BLOCK, 3) void swe_ens_mixed_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     SweEnsArgs A) {
   __shared__ F4x2 tile[kSweTile];
