@@ -248,6 +248,28 @@ def test_rk4_steps_match_oracle(ctx, ora, compat, renorm):
     assert rel_l2(aos(s.velocity_faces.cpu().numpy()), st["fu"]) < 1e-11
 
 
+def test_small_rh54_run_matches_oracle(ctx, ora):
+    # the CPU-runnable configuration: icosahedral level 4, RH 5-4
+    # (u0 = 0.1, amp = -0.1), Omega = 2 pi, eps = 0, dt = 0.01, 10 RK4 steps
+    # in the reference's stage layout (compat), every stored face a target
+    from lpm_amd.bve import BVERK4
+    s = _sphere(ctx, 0, 4)
+    s.init_velocity()
+    st = _oracle_state(s)
+    solver = BVERK4(0.01, s, reference_compat=True)
+    for _ in range(10):
+        solver.advance_timestep(s)
+        ora.bve_rk4_step(st, 0.01, OMEGA, compat=True)
+    torch.cuda.synchronize()
+    leaf = s.mesh.face_mask == 0
+    assert rel_l2(aos(s.vert_x.cpu().numpy()), st["vx"]) < TOL64
+    assert rel_l2(aos(s.face_x.cpu().numpy())[leaf], st["fx"][leaf]) < TOL64
+    assert rel_l2(s.rel_vort_verts.cpu().numpy(), st["vzeta"]) < TOL64
+    assert rel_l2(s.rel_vort_faces.cpu().numpy()[leaf], st["fzeta"][leaf]) < TOL64
+    assert rel_l2(aos(s.velocity_verts.cpu().numpy()), st["vu"]) < TOL64
+    assert rel_l2(aos(s.velocity_faces.cpu().numpy())[leaf], st["fu"][leaf]) < TOL64
+
+
 def test_rk4_graph_replays_match_eager(ctx):
     from lpm_amd.bve import BVERK4
     a = _sphere(ctx, 0, 3)

@@ -77,7 +77,7 @@ def test_bve_rotation_vtk_frames(tmp_path):
     exact = np.stack([-omg * x[:, 1], omg * x[:, 0], 0 * x[:, 0]], 1)
     u = f0["point_data"]["velocity"]
     assert np.linalg.norm(u - exact) / np.linalg.norm(exact) < 2e-2
-    # the last frame holds the rotated positions the driver reports
+    # the last frame holds the rotated positions the example program reports
     ang = omg * out["t"]
     c, s = np.cos(ang), np.sin(ang)
     xr = np.stack([c * x[:, 0] - s * x[:, 1], s * x[:, 0] + c * x[:, 1],
