@@ -457,6 +457,41 @@ def ic2d_run(ctx, steps=10, warmup=1, dt=0.005, level=6, oracle_stride=64):
     }
     log(f"incompressible2d: {el / steps * 1e3:.1f} ms/step, "
         f"{out['value']:.1f} Gpairs/s")
+    uT = s.velocity_passive.cpu().numpy()
+    del s, r
+
+    # mixed precision: same run with FP32 pairs and the native FP32 log
+    m = Incompressible2D("icos", level, ctx=ctx, mesh=mesh)
+    m.init_vorticity(lambda x: G.rh54_vorticity(x, 0.1, -0.1))
+    m.init_direct_sums(skip, precision="mixed")
+    torch.cuda.synchronize()
+    mu0 = m.velocity_passive.cpu().numpy()
+    mp0 = m.stream_fn_passive.cpu().numpy()
+    rm = Incompressible2DRK2(dt, m, leaf_only=True, renormalize=True,
+                             use_graph=True, stream=stream.cuda_stream,
+                             precision="mixed")
+    for _ in range(warmup):
+        m.advance_timestep(rm)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.advance_timestep(rm)
+    stream.synchronize()
+    elm = time.perf_counter() - t0
+    out["mixed"] = {
+        "value": round(2 * pairs * steps / elm / 1e9, 3),
+        "unit": "Gpair-interactions/s",
+        "ms_per_step": round(elm / steps * 1e3, 3),
+        "speedup_vs_fp64": round(el / elm, 4),
+        "dtype": "f32 pair arithmetic and log, f64 carries/state",
+        "velocity_vs_fp64_t0_rel_l2_linf": errors(mu0, u0),
+        "stream_fn_vs_fp64_t0_rel_l2_linf": errors1(mp0, p0),
+        "velocity_vs_fp64_after_steps_rel_l2_linf": errors(
+            m.velocity_passive.cpu().numpy(), uT),
+    }
+    log(f"incompressible2d mixed: {elm / steps * 1e3:.1f} ms/step, "
+        f"{out['mixed']['value']:.1f} Gpairs/s ({el / elm:.2f}x)")
+    del m, rm
     return out
 
 

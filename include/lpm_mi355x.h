@@ -250,15 +250,17 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
  * Incompressible2DReducer and Incompressible2DKernels<SphereGeometry>::
  * kernel_vals (src/lpm_incompressible2d_kernels.hpp:35-47, 96-247).
  * Collocated sums skip j == i only when eps == 0, as the reference does
- * (:242).  FP64 only.  Arguments as lpm_bve_velocity; out_u or out_psi may
- * be NULL. */
+ * (:242).  precision LPM_FP64, or LPM_MIXED (FP32 pairs on tile-relative
+ * offsets with the native FP32 log, FP64 tile carries; no reference
+ * counterpart).  Arguments as lpm_bve_velocity; out_u or out_psi may be
+ * NULL. */
 int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                   int64_t tgt_cs, int64_t n_tgt, const double* src_x,
                   int64_t src_es, int64_t src_cs, const double* src_zeta,
                   const double* src_area, const uint8_t* src_mask,
-                  int64_t n_src, int collocated, double eps, unsigned flags,
-                  double* out_u, int64_t out_es, int64_t out_cs,
-                  double* out_psi, void* stream);
+                  int64_t n_src, int collocated, double eps, int precision,
+                  unsigned flags, double* out_u, int64_t out_es,
+                  int64_t out_cs, double* out_psi, void* stream);
 
 /* Fields of Incompressible2D<Seed> (src/lpm_incompressible2d.hpp) the
  * stepper reads and writes. */
@@ -285,7 +287,8 @@ typedef struct lpm_ic2d_fields {
  * (Incompressible2D::init_direct_sums,
  * src/lpm_incompressible2d_impl.hpp:235-254). */
 int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
-                              double eps, unsigned flags, void* stream);
+                              double eps, int precision, unsigned flags,
+                              void* stream);
 
 /* One step of Incompressible2DRK2 (Heun's method,
  * src/lpm_incompressible2d_rk2_impl.hpp:75-172) with
@@ -295,7 +298,8 @@ int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
  * LPM_FLAG_SKIP_MASKED_TARGETS, LPM_RK4_RENORMALIZE (stage and final
  * positions projected onto the sphere), LPM_RK4_USE_GRAPH. */
 int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
-                      double Omega, double eps, unsigned flags, void* stream);
+                      double Omega, double eps, int precision, unsigned flags,
+                      void* stream);
 
 /* --------------------------------------------------------- SWE sums ---- */
 /* Sphere shallow-water direct sums at the targets: velocity from relative

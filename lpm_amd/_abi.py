@@ -171,11 +171,11 @@ _SIGS = {
                                           c_int, c_uint, c_vp]),
     "lpm_ic2d_sums": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
                               c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_dbl,
-                              c_uint, c_vp, c_i64, c_i64, c_vp, c_vp]),
+                              c_int, c_uint, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "lpm_ic2d_init_direct_sums": (c_int, [c_vp, ctypes.POINTER(Ic2dFields),
-                                          c_dbl, c_uint, c_vp]),
+                                          c_dbl, c_int, c_uint, c_vp]),
     "lpm_ic2d_rk2_step": (c_int, [c_vp, ctypes.POINTER(Ic2dFields), c_dbl,
-                                  c_dbl, c_dbl, c_uint, c_vp]),
+                                  c_dbl, c_dbl, c_int, c_uint, c_vp]),
     "lpm_variant_count": (c_int, [c_int]),
     "lpm_variant_default": (c_int, [c_int]),
     "lpm_variant_name": (ctypes.c_char_p, [c_int, c_int]),

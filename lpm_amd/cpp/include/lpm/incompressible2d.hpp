@@ -37,9 +37,10 @@ class Incompressible2D : public PolyMesh2d<SeedType> {
 
   // Incompressible2D::init_direct_sums
   // (src/lpm_incompressible2d_impl.hpp:235-254).
-  void init_direct_sums(unsigned flags = 0) {
+  void init_direct_sums(unsigned flags = 0, int precision = LPM_FP64) {
     lpm_ic2d_fields f = fields();
-    check(lpm_ic2d_init_direct_sums(ctx_->get(), &f, eps, flags, nullptr),
+    check(lpm_ic2d_init_direct_sums(ctx_->get(), &f, eps, precision, flags,
+                                    nullptr),
           ctx_->get());
   }
 
@@ -112,15 +113,15 @@ template <typename SeedType>
 class Incompressible2DRK2 {
  public:
   Incompressible2DRK2(const Real timestep, Incompressible2D<SeedType>& ic2d,
-                      unsigned flags = 0)
-      : dt(timestep), t_idx(0), flags_(flags) {
+                      unsigned flags = 0, int precision = LPM_FP64)
+      : dt(timestep), t_idx(0), flags_(flags), precision_(precision) {
     (void)ic2d;
   }
 
   void advance_timestep_impl(Incompressible2D<SeedType>& ic2d) {
     lpm_ic2d_fields f = ic2d.fields();
-    check(lpm_ic2d_rk2_step(ic2d.ctx(), &f, dt, ic2d.Omega, ic2d.eps, flags_,
-                            nullptr),
+    check(lpm_ic2d_rk2_step(ic2d.ctx(), &f, dt, ic2d.Omega, ic2d.eps,
+                            precision_, flags_, nullptr),
           ic2d.ctx());
     ++t_idx;
   }
@@ -130,6 +131,7 @@ class Incompressible2DRK2 {
 
  private:
   unsigned flags_;
+  int precision_;
 };
 
 }  // namespace Lpm

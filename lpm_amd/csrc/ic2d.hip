@@ -8,6 +8,12 @@
 //   arg = 1 - x.y + eps^2,  S += (s / arg) y,  psi += s log(arg),
 // and u = x cross S at the end (x cross y / (-4 pi arg) Gamma = x cross
 // (s / arg) y).  The FP64 log dominates the pair cost.
+//
+// Mixed precision: the pair runs in FP32 on tile-relative offsets
+// d = y' - x' (as the BVE mixed kernel), with 1 - x.y = |d|^2 / 2 on the
+// sphere, so arg = |d|^2 / 2 + eps^2, the weight 2 s / (|d|^2 + 2 eps^2)
+// and psi += s log(arg) with the native FP32 log; FP32 partials per
+// 256-source tile are carried into FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +68,65 @@ __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
       S[0][2] = __builtin_fma(w, y.z, S[0][2]);
       psi += p;
     }
+  }
+  if (sg.out) store_cross<1>(sg, x, S, live, tgt);
+  if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void ic2d_mixed_kernel(
+    const float4* __restrict__ src, const double4* __restrict__ centers,
+    const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, float e2x2,
+    float e2, int skip_self) {
+  __shared__ float4 tile[kPackTile];
+  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  const int nsrc = *count;
+  double x[1][3];
+  int self[1];
+  bool live[1];
+  int64_t tgt[1];
+  if (!load_targets<BLOCK, 1>(sg, blk, x, self, live, tgt)) return;
+  if (!skip_self) self[0] = -1;
+  double S[1][3] = {{0.0, 0.0, 0.0}};
+  double psi = 0.0;
+  constexpr float kLn2 = 0.693147180559945309f;
+  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kPackTile; t += BLOCK) {
+      const int j = j0 + t;
+      // padding: far away, zero strength -> contributes exactly 0
+      tile[t] = j < nsrc ? src[j] : make_float4(8.0f, 8.0f, 8.0f, 0.0f);
+    }
+    __syncthreads();
+    const double4 c = centers[j0 / kPackTile];
+    const float xr0 = static_cast<float>(x[0][0] - c.x);
+    const float xr1 = static_cast<float>(x[0][1] - c.y);
+    const float xr2 = static_cast<float>(x[0][2] - c.z);
+    const bool diag = static_cast<unsigned>(self[0] - j0) <
+                      static_cast<unsigned>(kPackTile);
+    float T0 = 0.0f, T1 = 0.0f, T2 = 0.0f, P = 0.0f;
+    for (int jj = 0; jj < kPackTile; ++jj) {
+      const float4 y = tile[jj];
+      const float d0 = y.x - xr0, d1 = y.y - xr1, d2 = y.z - xr2;
+      const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      float w = y.w * __builtin_amdgcn_rcpf(r2 + e2x2);
+      // y.w = 2 s: psi term s log(arg) = y.w log2(arg) (ln 2 / 2)
+      float p = y.w * __builtin_amdgcn_logf(__builtin_fmaf(0.5f, r2, e2));
+      if (diag && j0 + jj == self[0]) {
+        w = 0.0f;
+        p = 0.0f;
+      }
+      T0 = __builtin_fmaf(w, d0, T0);
+      T1 = __builtin_fmaf(w, d1, T1);
+      T2 = __builtin_fmaf(w, d2, T2);
+      P += p;
+    }
+    S[0][0] += static_cast<double>(T0);
+    S[0][1] += static_cast<double>(T1);
+    S[0][2] += static_cast<double>(T2);
+    psi += static_cast<double>(P) * (0.5 * static_cast<double>(kLn2));
   }
   if (sg.out) store_cross<1>(sg, x, S, live, tgt);
   if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
@@ -147,7 +212,8 @@ __global__ __launch_bounds__(256) void ic2d_update_kernel(Ic2dArgs A) {
 inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
 void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
-                 int nseg, double eps, bool collocated_skip, hipStream_t s) {
+                 int nseg, double eps, bool collocated_skip, int precision,
+                 hipStream_t s) {
   TargetSeg empty;
   const TargetSeg& t0 = segs[0];
   const TargetSeg& t1 = nseg > 1 ? segs[1] : empty;
@@ -174,18 +240,26 @@ void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
   SegArgs a0 = to_args(t0), a1 = to_args(t1);
   const double pairs = static_cast<double>(t0.n + t1.n) * src.capacity;
   TimedLaunch tl(ctx, KID_IC2D, s, pairs);
-  hipLaunchKernelGGL((ic2d_f64_kernel<kBlock>), dim3(a0.nblocks + a1.nblocks),
-                     dim3(kBlock), 0, s,
-                     static_cast<const double4*>(src.data), src.count, a0, a1,
-                     eps * eps, collocated_skip ? 1 : 0);
+  if (precision == LPM_MIXED) {
+    hipLaunchKernelGGL((ic2d_mixed_kernel<kBlock>),
+                       dim3(a0.nblocks + a1.nblocks), dim3(kBlock), 0, s,
+                       static_cast<const float4*>(src.data), src.centers,
+                       src.count, a0, a1, static_cast<float>(2.0 * eps * eps),
+                       static_cast<float>(eps * eps), collocated_skip ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL((ic2d_f64_kernel<kBlock>),
+                       dim3(a0.nblocks + a1.nblocks), dim3(kBlock), 0, s,
+                       static_cast<const double4*>(src.data), src.count, a0,
+                       a1, eps * eps, collocated_skip ? 1 : 0);
+  }
   LPM_HIP_CHECK(hipGetLastError());
 }
 
 // Sums of the fields at positions (vx, fx) with face vorticity fzeta.
 void ic2d_eval(lpm_ctx* ctx, const lpm_ic2d_fields& f, CrdIn vx, CrdIn fx,
                const double* fzeta, const RankMap& map, double eps,
-               unsigned flags, hipStream_t s) {
-  PackedSources ps = pack_sources(ctx, fx, fzeta, f.face_area, map, LPM_FP64, s);
+               int precision, unsigned flags, hipStream_t s) {
+  PackedSources ps = pack_sources(ctx, fx, fzeta, f.face_area, map, precision, s);
   TargetSeg segs[2];
   segs[0].x = vx;
   segs[0].n = f.n_verts;
@@ -196,7 +270,7 @@ void ic2d_eval(lpm_ctx* ctx, const lpm_ic2d_fields& f, CrdIn vx, CrdIn fx,
                          CrdOut{f.face_u, f.face_u_es, f.face_u_cs},
                          f.face_psi);
   // the self-exclusion of segment 1 applies only when eps == 0
-  launch_ic2d(ctx, ps, segs, 2, eps, eps == 0.0, s);
+  launch_ic2d(ctx, ps, segs, 2, eps, eps == 0.0, precision, s);
 }
 
 void validate(const lpm_ic2d_fields* f) {
@@ -213,8 +287,8 @@ void validate(const lpm_ic2d_fields* f) {
 }
 
 void issue_rk2(lpm_ctx* ctx, const lpm_ic2d_fields& f, double dt, double Omega,
-               double eps, unsigned flags, Ic2dStage& V, Ic2dStage& F,
-               hipStream_t s) {
+               double eps, int precision, unsigned flags, Ic2dStage& V,
+               Ic2dStage& F, hipStream_t s) {
   const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
   const uint8_t* fskip =
       (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? f.face_mask : nullptr;
@@ -242,7 +316,7 @@ void issue_rk2(lpm_ctx* ctx, const lpm_ic2d_fields& f, double dt, double Omega,
   ic2d_stage1_kernel<<<nblk(f.n_faces), 256, 0, s>>>(args(true));
   LPM_HIP_CHECK(hipGetLastError());
   ic2d_eval(ctx, f, CrdIn{V.xw, 1, V.n}, CrdIn{F.xw, 1, F.n}, F.zw, map, eps,
-            flags, s);
+            precision, flags, s);
   if (f.n_verts > 0) {
     ic2d_update_kernel<<<nblk(f.n_verts), 256, 0, s>>>(args(false));
     LPM_HIP_CHECK(hipGetLastError());
@@ -251,11 +325,12 @@ void issue_rk2(lpm_ctx* ctx, const lpm_ic2d_fields& f, double dt, double Omega,
   LPM_HIP_CHECK(hipGetLastError());
   ic2d_eval(ctx, f, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
             CrdIn{f.face_x, f.face_x_es, f.face_x_cs}, f.face_zeta, map, eps,
-            flags, s);
+            precision, flags, s);
 }
 
 std::string ic2d_key(const lpm_ic2d_fields& f, double dt, double Omega,
-                     double eps, unsigned flags, hipStream_t s) {
+                     double eps, int precision, unsigned flags,
+                     hipStream_t s) {
   std::ostringstream k;
   k << "ic2d|" << f.n_verts << '|' << f.vert_x << '|' << f.vert_x_es << '|'
     << f.vert_x_cs << '|' << f.vert_zeta << '|' << f.vert_u << '|'
@@ -265,7 +340,8 @@ std::string ic2d_key(const lpm_ic2d_fields& f, double dt, double Omega,
     << f.face_u_es << '|' << f.face_u_cs << '|' << f.face_psi << '|'
     << f.face_area << '|' << static_cast<const void*>(f.face_mask) << '|';
   k.precision(17);
-  k << dt << '|' << Omega << '|' << eps << '|' << flags << '|' << s;
+  k << dt << '|' << Omega << '|' << eps << '|' << precision << '|' << flags
+    << '|' << s;
   return k.str();
 }
 
@@ -294,12 +370,14 @@ int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                   int64_t tgt_cs, int64_t n_tgt, const double* src_x,
                   int64_t src_es, int64_t src_cs, const double* src_zeta,
                   const double* src_area, const uint8_t* src_mask,
-                  int64_t n_src, int collocated, double eps, unsigned flags,
-                  double* out_u, int64_t out_es, int64_t out_cs,
-                  double* out_psi, void* stream) {
+                  int64_t n_src, int collocated, double eps, int precision,
+                  unsigned flags, double* out_u, int64_t out_es,
+                  int64_t out_cs, double* out_psi, void* stream) {
   return guarded(ctx, [&] {
     LPM_ARG_CHECK(ctx, "null context");
     LPM_ARG_CHECK(n_tgt >= 0 && n_src >= 0, "negative size");
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
     LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
     if (n_tgt > 0) {
       LPM_ARG_CHECK(tgt_x && tgt_es > 0 && tgt_cs > 0, "bad tgt_x");
@@ -316,7 +394,7 @@ int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
     hipStream_t s = as_stream(stream);
     const RankMap map = build_rank(ctx, src_mask, n_src, s);
     PackedSources ps = pack_sources(ctx, CrdIn{src_x, src_es, src_cs}, src_zeta,
-                                    src_area, map, LPM_FP64, s);
+                                    src_area, map, precision, s);
     TargetSeg t;
     if (collocated) {
       t = face_targets(CrdIn{tgt_x, tgt_es, tgt_cs}, map, src_mask,
@@ -328,32 +406,38 @@ int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
       t.out = CrdOut{out_u, out_es, out_cs};
       t.out_scalar = out_psi;
     }
-    launch_ic2d(ctx, ps, &t, 1, eps, collocated && eps == 0.0, s);
+    launch_ic2d(ctx, ps, &t, 1, eps, collocated && eps == 0.0, precision, s);
     if ((flags & LPM_FLAG_CHECK_FINITE) && out_u)
       check_finite(ctx, out_u, n_tgt, out_es, out_cs, 3, s);
   });
 }
 
 int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
-                              double eps, unsigned flags, void* stream) {
+                              double eps, int precision, unsigned flags,
+                              void* stream) {
   return guarded(ctx, [&] {
     LPM_ARG_CHECK(ctx, "null context");
     validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
     LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
     ic2d_eval(ctx, *f, CrdIn{f->vert_x, f->vert_x_es, f->vert_x_cs},
               CrdIn{f->face_x, f->face_x_es, f->face_x_cs}, f->face_zeta, map,
-              eps, flags, s);
+              eps, precision, flags, s);
   });
 }
 
 int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
-                      double Omega, double eps, unsigned flags, void* stream) {
+                      double Omega, double eps, int precision, unsigned flags,
+                      void* stream) {
   return guarded(ctx, [&] {
     LPM_ARG_CHECK(ctx, "null context");
     validate(f);
+    LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                  "unknown precision");
     LPM_ARG_CHECK(std::isfinite(dt) && std::isfinite(Omega),
                   "dt and Omega must be finite");
     LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
@@ -366,10 +450,12 @@ int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
         [&]() {
           const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
           pack_sources(ctx, CrdIn{f->face_x, f->face_x_es, f->face_x_cs},
-                       f->face_zeta, f->face_area, map, LPM_FP64, s);
-          return ic2d_key(*f, dt, Omega, eps, flags, s);
+                       f->face_zeta, f->face_area, map, precision, s);
+          return ic2d_key(*f, dt, Omega, eps, precision, flags, s);
         },
-        [&]() { issue_rk2(ctx, *f, dt, Omega, eps, flags, V, F, s); });
+        [&]() {
+          issue_rk2(ctx, *f, dt, Omega, eps, precision, flags, V, F, s);
+        });
     if (flags & LPM_FLAG_CHECK_FINITE) {
       check_finite(ctx, f->vert_u, f->n_verts, f->vert_u_es, f->vert_u_cs, 3, s);
       check_finite(ctx, f->face_u, f->n_faces, f->face_u_es, f->face_u_cs, 3, s);

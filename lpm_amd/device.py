@@ -150,7 +150,7 @@ class Context:
     # --------------------------------------------------- Incompressible2D
     def ic2d_sums(self, tgt_x, src_x, src_zeta, src_area, src_mask=None,
                   collocated=False, eps=0.0, flags=0, out_u=None, out_psi=None,
-                  stream=None):
+                  stream=None, precision="fp64"):
         """Fused velocity (3, n) and stream function (n) at the targets."""
         tp, tes, tcs, nt = crd_args(tgt_x)
         sp, ses, scs, ns = crd_args(src_x)
@@ -162,19 +162,22 @@ class Context:
         self.check(self.L.lpm_ic2d_sums(
             self.h, tp, tes, tcs, nt, sp, ses, scs, _ptr(src_zeta),
             _ptr(src_area), _ptr(src_mask), ns, int(collocated), float(eps),
-            flags, op, oes, ocs, _ptr(out_psi),
+            precision_id(precision), flags, op, oes, ocs, _ptr(out_psi),
             stream if stream is not None else current_stream()))
         return out_u, out_psi
 
-    def ic2d_init_direct_sums(self, fields, eps, flags=0, stream=None):
+    def ic2d_init_direct_sums(self, fields, eps, flags=0, stream=None,
+                              precision="fp64"):
         self.check(self.L.lpm_ic2d_init_direct_sums(
-            self.h, ctypes.byref(fields), float(eps), flags,
-            stream if stream is not None else current_stream()))
+            self.h, ctypes.byref(fields), float(eps), precision_id(precision),
+            flags, stream if stream is not None else current_stream()))
 
-    def ic2d_rk2_step(self, fields, dt, Omega, eps, flags=0, stream=None):
+    def ic2d_rk2_step(self, fields, dt, Omega, eps, flags=0, stream=None,
+                      precision="fp64"):
         self.check(self.L.lpm_ic2d_rk2_step(
             self.h, ctypes.byref(fields), float(dt), float(Omega), float(eps),
-            flags, stream if stream is not None else current_stream()))
+            precision_id(precision), flags,
+            stream if stream is not None else current_stream()))
 
     # ----------------------------------------------------------- SWE sums
     def swe_sums(self, tgt_x, tgt_surf, src_x, src_zeta, src_sigma, src_area,

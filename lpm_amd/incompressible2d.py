@@ -65,10 +65,12 @@ class Incompressible2D:
         f.face_mask = self.face_mask.data_ptr()
         return f
 
-    def init_direct_sums(self, flags=0):
+    def init_direct_sums(self, flags=0, precision="fp64"):
         """Incompressible2D::init_direct_sums
-        (src/lpm_incompressible2d_impl.hpp:235-254)."""
-        self.ctx.ic2d_init_direct_sums(self.fields(), self.eps, flags)
+        (src/lpm_incompressible2d_impl.hpp:235-254); precision "fp64" or
+        "mixed"."""
+        self.ctx.ic2d_init_direct_sums(self.fields(), self.eps, flags,
+                                       precision=precision)
 
     def advance_timestep(self, solver: "Incompressible2DRK2"):
         solver.advance_timestep_impl(self)
@@ -80,9 +82,11 @@ class Incompressible2DRK2:
     state (2 evaluations per step)."""
 
     def __init__(self, dt: float, ic2d: Incompressible2D, leaf_only=False,
-                 renormalize=False, use_graph=False, stream=None):
+                 renormalize=False, use_graph=False, stream=None,
+                 precision="fp64"):
         self.dt = float(dt)
         self.t_idx = 0
+        self.precision = precision
         flags = 0
         if leaf_only:
             flags |= _abi.LPM_FLAG_SKIP_MASKED_TARGETS
@@ -98,5 +102,6 @@ class Incompressible2DRK2:
         if self._fields is None:
             self._fields = ic2d.fields()
         ic2d.ctx.ic2d_rk2_step(self._fields, self.dt, ic2d.Omega, ic2d.eps,
-                               self.flags, self.stream)
+                               self.flags, self.stream,
+                               precision=self.precision)
         self.t_idx += 1

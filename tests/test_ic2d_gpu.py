@@ -3,7 +3,7 @@ function) and the Incompressible2DRK2 stepper against the FP64 oracle.
 
 Tolerances: 1e-12 relative for the sums (FP64 throughout; tiles, the
 factored cross product and the Newton reciprocal reorder the arithmetic),
-1e-11 after RK2 steps.
+1e-11 after RK2 steps; the mixed path stays within 1e-6 of FP64.
 """
 import numpy as np
 import pytest
@@ -116,9 +116,57 @@ def test_argument_errors(ctx):
     one = dev(np.ones(3))
     p = x.data_ptr()
     rc = L.lpm_ic2d_sums(ctx.h, p, 1, 3, 3, p, 1, 3, one.data_ptr(),
-                         one.data_ptr(), None, 2, 1, 0.0, 0, p, 1, 3, None, None)
+                         one.data_ptr(), None, 2, 1, 0.0, 0, 0, p, 1, 3, None,
+                         None)
     assert rc == _abi.LPM_ERR_ARG
     assert b"collocated" in L.lpm_last_error(ctx.h)
     rc = L.lpm_ic2d_sums(ctx.h, p, 1, 3, 3, p, 1, 3, one.data_ptr(),
-                         one.data_ptr(), None, 3, 0, -1.0, 0, p, 1, 3, None, None)
+                         one.data_ptr(), None, 3, 0, -1.0, 0, 0, p, 1, 3, None,
+                         None)
     assert rc == _abi.LPM_ERR_ARG
+    rc = L.lpm_ic2d_sums(ctx.h, p, 1, 3, 3, p, 1, 3, one.data_ptr(),
+                         one.data_ptr(), None, 3, 0, 0.0, 7, 0, p, 1, 3, None,
+                         None)
+    assert rc == _abi.LPM_ERR_ARG
+    assert b"precision" in L.lpm_last_error(ctx.h)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_mixed_sums_match_fp64(ctx, seed, eps):
+    # FP32 pairs on tile-relative offsets, native FP32 log, FP64 carries
+    m = mesh(seed, 4)
+    z = G.rh54_vorticity(m.face_x, 0.1, -0.1)
+    mask = dev(m.face_mask, torch.uint8)
+    leaf = m.face_mask == 0
+    for tgt, col in ((dev(m.vert_x), False), (dev(m.face_x), True)):
+        u64, p64 = ctx.ic2d_sums(tgt, dev(m.face_x), dev(z), dev(m.face_area),
+                                 mask, collocated=col, eps=eps)
+        u32, p32 = ctx.ic2d_sums(tgt, dev(m.face_x), dev(z), dev(m.face_area),
+                                 mask, collocated=col, eps=eps,
+                                 precision="mixed")
+        a, b = u32.cpu().numpy(), u64.cpu().numpy()
+        pa, pb = p32.cpu().numpy(), p64.cpu().numpy()
+        if col:
+            a, b, pa, pb = a[:, leaf], b[:, leaf], pa[leaf], pb[leaf]
+        assert rel_l2(a, b) < 1e-6, (col, rel_l2(a, b))
+        assert rel_l2(pa, pb) < 1e-6, (col, rel_l2(pa, pb))
+
+
+def test_mixed_rk2_tracks_fp64(ctx):
+    from lpm_amd.incompressible2d import Incompressible2DRK2
+    a = _model(ctx, 0, 4, 0.0, 2 * np.pi)
+    b = _model(ctx, 0, 4, 0.0, 2 * np.pi)
+    a.init_direct_sums(flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    b.init_direct_sums(flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS, precision="mixed")
+    ra = Incompressible2DRK2(0.01, a, leaf_only=True, renormalize=True)
+    rb = Incompressible2DRK2(0.01, b, leaf_only=True, renormalize=True,
+                             precision="mixed")
+    for _ in range(5):
+        a.advance_timestep(ra)
+        b.advance_timestep(rb)
+    torch.cuda.synchronize()
+    assert rel_l2(b.velocity_passive.cpu().numpy(),
+                  a.velocity_passive.cpu().numpy()) < 1e-6
+    # positions integrate the ~1e-7 velocity difference: dt |u| 5 steps
+    assert rel_l2(b.vert_x.cpu().numpy(), a.vert_x.cpu().numpy()) < 1e-8
