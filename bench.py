@@ -212,7 +212,8 @@ def ensemble_run(ctx, steps, warmup, dt=0.01, autotune=True):
     return out
 
 
-def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8):
+def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8,
+                    oracle_stride=256):
     """Gaussian vortex on the cubed sphere, level 8 (393,216 faces): mixed
     against FP64 velocity and positions after 1 and 50 RK4 steps."""
     import numpy as np
@@ -239,6 +240,9 @@ def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8):
         s = BVESphere("cube", level, ctx=ctx, mesh=mesh)
         init(s)
         s.init_velocity(prec, flags=2)
+        if prec == "fp64":
+            torch.cuda.synchronize()
+            u_t0 = s.velocity_verts.cpu().numpy()
         solver = BVERK4(dt, s, precision=prec, reference_compat=True,
                         renormalize=True, leaf_only=True)
         torch.cuda.synchronize()
@@ -266,6 +270,18 @@ def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8):
         out[f"after_{n}_steps"] = {
             "velocity_rel_l2_linf": errors(um, uf),
             "positions_rel_l2_linf": errors(xm, xf)}
+    # FP64 initial velocity against the CPU oracle on a sample of vertices
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sl = slice(0, mesh.n_verts, oracle_stride)
+    zf = G.gaussian_vortex_vorticity(mesh.face_x, ctr, 4 * np.pi, 4.0) + noise_f
+    ru = oracle.bve_velocity(np.ascontiguousarray(mesh.vert_x[:, sl].T),
+                             np.ascontiguousarray(mesh.face_x.T), zf,
+                             mesh.face_area, mesh.face_mask,
+                             nthreads=min(16, os.cpu_count() or 1))
+    out["fp64_vs_cpu_oracle_sample_t0"] = {
+        "targets": f"every {oracle_stride}th vertex",
+        "velocity_rel_l2_linf": errors(u_t0[:, sl], ru.T)}
     return out
 
 
@@ -495,6 +511,71 @@ def ic2d_run(ctx, steps=10, warmup=1, dt=0.005, level=6, oracle_stride=64):
     return out
 
 
+def stream_fn_run(ctx, mesh, reps=3, oracle_stride=512):
+    """BVE stream function (BVESphere::init_stream_fn): the FP64 sum of
+    s log(1 - x.y) at every vertex and leaf face of the workload mesh, one
+    log per pair (table-driven FP64 log)."""
+    import numpy as np
+    import torch
+    from lpm_amd import _abi
+    from lpm_amd import galleries as G
+
+    def dev(a, dtype=torch.float64):
+        return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+    z = G.rh54_vorticity(mesh.face_x, 0.1, -0.1)
+    vx, fx = dev(mesh.vert_x), dev(mesh.face_x)
+    fz, fa = dev(z), dev(mesh.face_area)
+    fm = dev(mesh.face_mask, torch.uint8)
+    pv = torch.zeros(mesh.n_verts, dtype=torch.float64, device="cuda")
+    pf = torch.zeros(mesh.n_faces, dtype=torch.float64, device="cuda")
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+
+    def call():
+        ctx.bve_stream_fn(vx, fx, fz, fa, fm, out=pv)
+        ctx.bve_stream_fn(fx, fx, fz, fa, fm, collocated=True, flags=skip,
+                          out=pf)
+    call()
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    kms, _, _ = ctx.timing_query(_abi.LPM_KERNEL_BVE_PSI)
+    ctx.timing_enable(False)
+    pairs = mesh.useful_pairs()
+    psi_v = pv.cpu().numpy()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sl = slice(0, mesh.n_verts, oracle_stride)
+    ref = oracle.bve_stream_fn(np.ascontiguousarray(mesh.vert_x[:, sl].T),
+                               np.ascontiguousarray(mesh.face_x.T), z,
+                               mesh.face_area, mesh.face_mask,
+                               nthreads=min(16, os.cpu_count() or 1))
+    out = {
+        "workload": (f"BVE stream function, icosahedral mesh level "
+                     f"{mesh.depth} ({mesh.n_verts} vertices + "
+                     f"{mesh.n_leaves} faces), RH 5-4, FP64 with one log "
+                     f"per pair"),
+        "pairs_per_evaluation": pairs,
+        "value": round(pairs / el / 1e9, 3),
+        "unit": "Gpair-interactions/s",
+        "ms_per_evaluation": round(el * 1e3, 3),
+        "kernel_ms_per_evaluation": round(kms / reps, 3),
+        "dtype": "f64",
+        "vs_analytic_rh54_vertices_rel_l2_linf": errors1(
+            psi_v, G.rh54_stream_function(mesh.vert_x, 0.1, -0.1)),
+        "fp64_vs_cpu_oracle_sample": {
+            "targets": f"every {oracle_stride}th vertex",
+            "stream_fn_rel_l2_linf": errors1(psi_v[sl], ref)},
+    }
+    log(f"stream function: {el * 1e3:.1f} ms/evaluation, "
+        f"{pairs / el / 1e9:.1f} Gpairs/s")
+    return out
+
+
 def cpu_baseline(mesh, z_faces, budget_s=12.0):
     """Time the oracle (reference-faithful loop: every stored face, masked
     ones skipped) on a bounded sample of vertex targets."""
@@ -601,6 +682,8 @@ def main(argv=None):
                     help="skip the 16-state shallow-water ensemble")
     ap.add_argument("--no-ic2d", action="store_true",
                     help="skip the Incompressible2D run")
+    ap.add_argument("--no-stream-fn", action="store_true",
+                    help="skip the BVE stream-function evaluation")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -743,6 +826,9 @@ def main(argv=None):
         log(f"mixed: {mstep*1e3:.1f} ms/step, {mval:.1f} Gpairs/s "
             f"({mval/value:.2f}x FP64)")
 
+    if not a.no_stream_fn:
+        log("stream function")
+        result["stream_fn"] = stream_fn_run(ctx, mesh)
     if not a.no_ensemble:
         log("ensemble run (64 members)")
         result["ensemble"] = ensemble_run(ctx, a.ensemble_steps, 1,

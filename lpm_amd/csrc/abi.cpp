@@ -89,7 +89,8 @@ std::string tune_key(const lpm_ctx* ctx, int family, int64_t n_targets) {
 int pick_variant(lpm_ctx* ctx, int family, int64_t n_targets) {
   if (ctx->forced[family] >= 0) return ctx->forced[family];
   auto it = ctx->tuned.find(tune_key(ctx, family, n_targets));
-  if (it != ctx->tuned.end() && it->second < variant_count(family))
+  if (it != ctx->tuned.end() && it->second >= 0 &&
+      it->second < variant_count(family))
     return it->second;
   return default_variant(family);
 }

@@ -1,5 +1,5 @@
 // Device building blocks of the BVE direct sums, shared by the one-problem
-// kernels (bve_sums.hip) and the ensemble kernels (bve_batch.hip): target
+// kernels (bve_sums.hip) and the ensemble kernels (bve_ensemble.hip): target
 // segment arguments, the pair bodies, and the per-block loops over LDS tiles
 // of packed sources.  The loops replace the per-target team reductions of
 // BVEVertexVelocity / BVEFaceVelocity (src/lpm_bve_sphere_kernels.hpp:
@@ -60,6 +60,63 @@ __device__ __forceinline__ double rcp_nr(double d) {
     r = __builtin_fma(r, e, r);
   }
   return r;
+}
+
+// FP64 natural logarithm, table-driven after P. T. P. Tang, "Table-driven
+// implementation of the logarithm function in IEEE floating-point
+// arithmetic", ACM TOMS 16(4), 1990.  With a = m 2^e, m in [1/2, 1) and j
+// the seven fraction bits of m below its leading one, c_j = 2 / (1 + (j +
+// 1/2) / 128) brings m near 1:
+//   r = m c_j - 1 (one FMA, |r| < 2^-7),
+//   log a = k ln 2 + t_j + log1p(r),  t_j = -log(c_j) + (e - k) ln 2,
+// log1p(r) by its Taylor series to r^8 (truncation below 2^-56 |r|).
+// k = e - 1 for m < 3/4 (j < 64) and k = e above, so that a just below a
+// power of two does not cancel k ln 2 against t_j; the end intervals use
+// c_0 = 2 and c_127 = 1 (t = 0), exact on either side of 1.  ln 2 =
+// kLn2Hi + kLn2Lo with kLn2Hi carrying 42 significant bits, so k kLn2Hi
+// is exact for |k| < 2^11.  Error within 1.3 ulp against an extended-
+// precision log over the double range; a <= 0, inf and NaN give log's
+// special values.  The table lives in LDS (2 KiB per block).
+constexpr int kLogTable = 128;
+constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;
+constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
+
+// Fills tab[j] = {c_j, t_j} with the block's threads; the caller
+// synchronises before the first log_f64.
+__device__ __forceinline__ void fill_log_table(double2* tab) {
+  for (int j = threadIdx.x; j < kLogTable; j += blockDim.x) {
+    double c = 2.0 / (1.0 + (j + 0.5) / kLogTable);
+    if (j == 0) c = 2.0;
+    if (j == kLogTable - 1) c = 1.0;
+    tab[j] = make_double2(c, -log(j < kLogTable / 2 ? 0.5 * c : c));
+  }
+}
+
+// SPECIAL = false leaves out the zero / negative / infinite cases (their
+// results are then finite garbage) for callers that screen arguments
+// themselves; NaN still propagates.
+template <bool SPECIAL = true>
+__device__ __forceinline__ double log_f64(double a, const double2* tab) {
+  const double m = __builtin_amdgcn_frexp_mant(a);
+  const int e = __builtin_amdgcn_frexp_exp(a);
+  const int j = (__double2hiint(m) >> 13) & (kLogTable - 1);
+  const double2 ct = tab[j];
+  const double k = static_cast<double>(e - 1 + (j >> 6));
+  const double r = __builtin_fma(m, ct.x, -1.0);
+  double q = -1.0 / 8.0;
+  q = __builtin_fma(q, r, 1.0 / 7.0);
+  q = __builtin_fma(q, r, -1.0 / 6.0);
+  q = __builtin_fma(q, r, 1.0 / 5.0);
+  q = __builtin_fma(q, r, -1.0 / 4.0);
+  q = __builtin_fma(q, r, 1.0 / 3.0);
+  q = __builtin_fma(q, r, -1.0 / 2.0);
+  const double hi = __builtin_fma(k, kLn2Hi, ct.y);
+  const double lo = __builtin_fma(r * r, q, k * kLn2Lo);
+  double v = hi + (r + lo);
+  if (SPECIAL && !(a > 0.0 && a < __builtin_inf()))
+    v = a == 0.0 ? -__builtin_inf()
+                 : (a == __builtin_inf() ? a : __builtin_nan(""));
+  return v;
 }
 
 // One source: S += s / (1 - x.y [+ eps^2]) * y.
@@ -745,7 +802,9 @@ __global__ __launch_bounds__(kPackTile) void pack_mfma16_kernel(
   tt[48] = static_cast<float>(ws);
 }
 
-template <bool EPS>
+// NACC: independent weight-product accumulators per target block (1 or 2;
+// two shorten the dependent MFMA chain).
+template <bool EPS, int NACC = 1>
 __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
                                              const SegArgs& sg, int blk,
                                              float eps2x2, float4* ly,
@@ -827,7 +886,11 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
                           : make_uint4(g[0], g[1], g[2], g[3]);
       B[tb] = __builtin_bit_cast(mh8, sel);
     }
-    mf4 D2[2] = {mf4{0.0f, 0.0f, 0.0f, 0.0f}, mf4{0.0f, 0.0f, 0.0f, 0.0f}};
+    mf4 D2[2][NACC];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int q = 0; q < NACC; ++q) D2[tb][q] = mf4{0.0f, 0.0f, 0.0f, 0.0f};
     const int c4 = lane & 3;
 #pragma unroll 1
     for (int ch = 0; ch < kPackTile / 32; ++ch) {
@@ -842,8 +905,8 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
             A, B[tb], mf16{}, 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 16; ++v)
-          D2[tb] = __builtin_amdgcn_mfma_f32_4x4x1f32(
-              tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb], 0, 0, 0);
+          D2[tb][v % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+              tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb][v % NACC], 0, 0, 0);
       }
     }
     // the two source halves (lanes l, l ^ 32) of each column; lane l keeps
@@ -851,7 +914,12 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
     float o[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      float v0s = D2[0][a], v1s = D2[1][a];
+      float v0s = D2[0][0][a], v1s = D2[1][0][a];
+#pragma unroll
+      for (int q = 1; q < NACC; ++q) {
+        v0s += D2[0][q][a];
+        v1s += D2[1][q][a];
+      }
       v0s += __shfl_xor(v0s, 32, 64);
       v1s += __shfl_xor(v1s, 32, 64);
       o[a] = h ? v1s : v0s;

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void bve_vel_mfma_kernel(
 }
 
 // MINW: minimum waves per SIMD asked of the register allocator (1: none).
-template <bool EPS, int MINW = 1>
+template <bool EPS, int MINW = 1, int NACC = 1>
 __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
     Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
     float eps2x2) {
@@ -184,16 +184,50 @@ __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  mfma16_block<EPS>(src, *count, sg, blk, eps2x2, ly, ls, la, lt);
+  mfma16_block<EPS, NACC>(src, *count, sg, blk, eps2x2, ly, ls, la, lt);
 }
 
 // ------------------------------------------------------ stream function ---
-// psi_i = sum_j s_j log(1 - x_i.y_j); FP64 log from the device math library.
+// psi_i = sum_j s_j log(1 - x_i.y_j) with the table-driven FP64 log.
+// CHECK drops the pair jj == self (tile-relative); only tiles holding some
+// lane's own source of the wave need it.  SPECIAL evaluates log's zero /
+// negative cases; without it the smallest argument is returned in amin.
+template <bool CHECK, bool SPECIAL, int TILE>
+__device__ __forceinline__ double psi_tile_f64(const double4* tile, double x0,
+                                               double x1, double x2, int self,
+                                               const double2* ltab,
+                                               double& amin) {
+  double psi = 0.0;
+  for (int jj = 0; jj < TILE; ++jj) {
+    const double4 y = tile[jj];
+    const double d = __builtin_fma(
+        -x2, y.z, __builtin_fma(-x1, y.y, __builtin_fma(-x0, y.x, 1.0)));
+    if (!SPECIAL) amin = fmin(amin, d);
+    const double v = y.w * log_f64<SPECIAL>(d, ltab);
+    psi += (CHECK && jj == self) ? 0.0 : v;
+  }
+  return psi;
+}
+
+// Redone with log's special cases in the rare waves where some argument
+// was <= 0 (coincident points).
+template <bool CHECK, int TILE>
+__device__ __forceinline__ double psi_tile_sum(const double4* tile, double x0,
+                                               double x1, double x2, int self,
+                                               const double2* ltab) {
+  double amin = 1.0;
+  double p = psi_tile_f64<CHECK, false, TILE>(tile, x0, x1, x2, self, ltab, amin);
+  if (__ballot(amin <= 0.0))
+    p = psi_tile_f64<CHECK, true, TILE>(tile, x0, x1, x2, self, ltab, amin);
+  return p;
+}
+
 template <int BLOCK, int TILE>
 __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
     SegArgs s0, SegArgs s1) {
   __shared__ double4 tile[TILE];
+  __shared__ double2 ltab[kLogTable];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
@@ -217,6 +251,7 @@ __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
     }
   }
   double psi = 0.0;
+  fill_log_table(ltab);
   for (int j0 = 0; j0 < nsrc; j0 += TILE) {
     __syncthreads();
     for (int t = threadIdx.x; t < TILE; t += BLOCK) {
@@ -224,12 +259,11 @@ __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
       tile[t] = j < nsrc ? src[j] : make_double4(0.0, 0.0, 0.0, 0.0);
     }
     __syncthreads();
-    for (int jj = 0; jj < TILE; ++jj) {
-      const double4 y = tile[jj];
-      const double t = x0 * y.x + x1 * y.y + x2 * y.z;
-      const double v = y.w * log(1.0 - t);
-      psi += (j0 + jj == self) ? 0.0 : v;
-    }
+    const int so = self < 0 ? -1 : self - j0;
+    if (__ballot(static_cast<unsigned>(so) < static_cast<unsigned>(TILE)))
+      psi += psi_tile_sum<true, TILE>(tile, x0, x1, x2, so, ltab);
+    else
+      psi += psi_tile_sum<false, TILE>(tile, x0, x1, x2, so, ltab);
   }
   if (live) sg.out_scalar[i] = psi;
 }
@@ -314,6 +348,9 @@ const MixedVariant kMixedVariants[] = {
      bve_vel_mfma16_kernel<false>, bve_vel_mfma16_kernel<true>},  // 8: f16 split
     {nullptr, nullptr, nullptr, nullptr, 256, 1, "mfma16_occ5_block256_near3",
      bve_vel_mfma16_kernel<false, 5>, bve_vel_mfma16_kernel<true, 5>},  // 9
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_block256_near3", bve_vel_mfma16_kernel<false, 5, 2>,
+     bve_vel_mfma16_kernel<true, 5, 2>},                                 // 10
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);

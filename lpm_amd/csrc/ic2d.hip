@@ -7,7 +7,8 @@
 // {y, s = -zeta A / 4 pi}) and target segments.  Per pair:
 //   arg = 1 - x.y + eps^2,  S += (s / arg) y,  psi += s log(arg),
 // and u = x cross S at the end (x cross y / (-4 pi arg) Gamma = x cross
-// (s / arg) y).  The FP64 log dominates the pair cost.
+// (s / arg) y).  The table-driven FP64 log (bve_device.hpp) dominates the
+// pair cost.
 //
 // Mixed precision: the pair runs in FP32 on tile-relative offsets
 // d = y' - x' (as the BVE mixed kernel), with 1 - x.y = |d|^2 / 2 on the
@@ -27,11 +28,63 @@
 namespace lpm_amd {
 namespace {
 
+// One LDS tile of FP64 sources against this lane's target, summed into
+// S / psi.  CHECK drops the pair jj == self (tile-relative); only tiles
+// holding some lane's own source of the wave need it.  SPECIAL evaluates
+// log's zero / negative cases; without it the smallest argument is
+// returned so that the caller can redo the tile when one was <= 0.
+template <bool CHECK, bool SPECIAL>
+__device__ __forceinline__ double ic2d_tile_f64(const double4* tile,
+                                                const double* x, double one_e,
+                                                int self, const double2* ltab,
+                                                double* S, double& psi) {
+  double amin = 1.0;
+  for (int jj = 0; jj < kPackTile; ++jj) {
+    const double4 y = tile[jj];
+    // 1 - x.y + eps^2 as one FMA chain from 1 + eps^2
+    const double arg = __builtin_fma(
+        -x[2], y.z, __builtin_fma(-x[1], y.y, __builtin_fma(-x[0], y.x, one_e)));
+    if (!SPECIAL) amin = fmin(amin, arg);
+    double w = y.w * rcp_nr<2>(arg);
+    double p = y.w * log_f64<SPECIAL>(arg, ltab);
+    if (CHECK && jj == self) {
+      w = 0.0;
+      p = 0.0;
+    }
+    S[0] = __builtin_fma(w, y.x, S[0]);
+    S[1] = __builtin_fma(w, y.y, S[1]);
+    S[2] = __builtin_fma(w, y.z, S[2]);
+    psi += p;
+  }
+  return amin;
+}
+
+// Per-tile partial sums, redone with log's special cases in the rare waves
+// where some argument was <= 0 (coincident points without smoothing).
+template <bool CHECK>
+__device__ __forceinline__ void ic2d_tile_sum(const double4* tile,
+                                              const double* x, double one_e,
+                                              int self, const double2* ltab,
+                                              double* S, double& psi) {
+  double T[3] = {0.0, 0.0, 0.0}, P = 0.0;
+  const double amin =
+      ic2d_tile_f64<CHECK, false>(tile, x, one_e, self, ltab, T, P);
+  if (__ballot(amin <= 0.0)) {
+    T[0] = T[1] = T[2] = P = 0.0;
+    ic2d_tile_f64<CHECK, true>(tile, x, one_e, self, ltab, T, P);
+  }
+  S[0] += T[0];
+  S[1] += T[1];
+  S[2] += T[2];
+  psi += P;
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
     SegArgs s0, SegArgs s1, double eps2, int skip_self) {
   __shared__ double4 tile[kPackTile];
+  __shared__ double2 ltab[kLogTable];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
@@ -44,6 +97,7 @@ __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
   if (!skip_self) self[0] = -1;
   double S[1][3] = {{0.0, 0.0, 0.0}};
   double psi = 0.0;
+  fill_log_table(ltab);
   for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
     __syncthreads();
     for (int t = threadIdx.x; t < kPackTile; t += BLOCK) {
@@ -51,23 +105,11 @@ __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
       tile[t] = j < nsrc ? src[j] : make_double4(0.0, 0.0, 0.0, 0.0);
     }
     __syncthreads();
-    const bool diag = static_cast<unsigned>(self[0] - j0) <
-                      static_cast<unsigned>(kPackTile);
-    for (int jj = 0; jj < kPackTile; ++jj) {
-      const double4 y = tile[jj];
-      const double t = x[0][0] * y.x + x[0][1] * y.y + x[0][2] * y.z;
-      const double arg = 1.0 - t + eps2;
-      double w = y.w * rcp_nr<2>(arg);
-      double p = y.w * log(arg);
-      if (diag && j0 + jj == self[0]) {
-        w = 0.0;
-        p = 0.0;
-      }
-      S[0][0] = __builtin_fma(w, y.x, S[0][0]);
-      S[0][1] = __builtin_fma(w, y.y, S[0][1]);
-      S[0][2] = __builtin_fma(w, y.z, S[0][2]);
-      psi += p;
-    }
+    const int so = self[0] < 0 ? -1 : self[0] - j0;
+    if (__ballot(static_cast<unsigned>(so) < static_cast<unsigned>(kPackTile)))
+      ic2d_tile_sum<true>(tile, x[0], 1.0 + eps2, so, ltab, S[0], psi);
+    else
+      ic2d_tile_sum<false>(tile, x[0], 1.0 + eps2, so, ltab, S[0], psi);
   }
   if (sg.out) store_cross<1>(sg, x, S, live, tgt);
   if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;

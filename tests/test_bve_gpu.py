@@ -130,6 +130,35 @@ def test_stream_function_matches_oracle(ctx, ora):
     assert rel_l2(psi.cpu().numpy()[leaf], ref[leaf]) < TOL64
 
 
+def test_stream_function_log_accuracy_and_special_values(ctx):
+    # one source at the pole: psi_i = s log(1 - z_i) with s = -zeta A / 4 pi,
+    # so the kernel's table-driven FP64 log is compared directly against the
+    # host libm log of the same double argument (1 - z is rounded once on
+    # both sides), over arguments from 1e-15 to 1e300 (1 - z cannot reach
+    # below 2^-53), both sides of 1 and of powers of two, and the
+    # coincident (log 0 = -inf) and just-past-the-pole (NaN) cases
+    rng = np.random.default_rng(5)
+    arg = np.concatenate([
+        10.0 ** rng.uniform(-15, 300, 4000), rng.uniform(0, 2, 4000),
+        1 + rng.uniform(-1e-6, 1e-6, 500), 2.0 ** np.arange(-53, 1000, 7.0),
+        1 - 2.0 ** -np.arange(1, 53.0), 1 + 2.0 ** -np.arange(1, 53.0),
+        np.nextafter(2.0 ** np.arange(-40, 40.0), 0)])
+    z = np.concatenate([1 - arg, [1.0, 1.0 + 2.0 ** -52]])
+    x = np.zeros((3, len(z)))
+    x[2] = z
+    y = np.array([[0.0], [0.0], [1.0]])
+    zeta, area = np.array([-4 * np.pi]), np.array([1.0])
+    psi = ctx.bve_stream_fn(dev(x), dev(y), dev(zeta), dev(area),
+                            None).cpu().numpy()
+    s = -zeta[0] * area[0] / (4 * np.pi)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        want = s * np.log(1 - z)
+    fin = np.isfinite(want)
+    np.testing.assert_allclose(psi[fin], want[fin], rtol=1e-15, atol=0)
+    assert psi[-2] == -np.inf and np.isnan(psi[-1])
+    assert fin[:-2].all()
+
+
 def test_skip_masked_targets_leaves_them_untouched(ctx):
     m, z = _inputs(0, 3)
     fx = dev(m.face_x)
@@ -486,6 +515,27 @@ def test_autotune_records_a_variant(ctx, tmp_path):
     ctx.tune_save(path)
     assert "gfx950" in open(path).read() or len(open(path).read()) > 0
     ctx.tune_load(path)
+    # a table entry naming no compiled variant falls back to the default
+    m, z = _inputs(0, 3)
+    args = (dev(m.vert_x), dev(m.face_x), dev(z), dev(m.face_area),
+            dev(m.face_mask, torch.uint8))
+    for kid in (_abi.LPM_KERNEL_BVE_VEL_F64, _abi.LPM_KERNEL_BVE_VEL_MIXED):
+        ctx.set_variant(kid, ctx.L.lpm_variant_default(kid))
+    want = ctx.bve_velocity(*args).cpu()
+    for kid in (_abi.LPM_KERNEL_BVE_VEL_F64, _abi.LPM_KERNEL_BVE_VEL_MIXED):
+        ctx.set_variant(kid, -1)
+    # keys are arch/family/size-bucket: poison every bucket of both families
+    arch = open(path).readline().split()[0].rsplit("/", 2)[0]
+    fams = {ln.split()[0].rsplit("/", 2)[1] for ln in open(path) if ln.strip()}
+    bad = str(tmp_path / "bad.txt")
+    with open(bad, "w") as f:
+        for fam in fams:
+            for b in range(63):
+                f.write(f"{arch}/{fam}/{b} {-3 if b % 2 else 999}\n")
+    ctx.tune_load(bad)
+    got = ctx.bve_velocity(*args).cpu()
+    ctx.tune_load(path)
+    assert torch.equal(got, want)
 
 
 def test_timing_records_launches(ctx):
