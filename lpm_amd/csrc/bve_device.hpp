@@ -802,6 +802,105 @@ __global__ __launch_bounds__(kPackTile) void pack_mfma16_kernel(
   tt[48] = static_cast<float>(ws);
 }
 
+// Difference form d = y' - x' on the vector ALUs over n sources of a tile
+// (near tiles), skipping local source `self` (any value outside [0, n):
+// none).
+template <bool EPS>
+__device__ __forceinline__ void mfma16_near(const float4* ly, const float* ls,
+                                            int n, float xr0, float xr1,
+                                            float xr2, float eps2x2, int self,
+                                            float& T0, float& T1, float& T2) {
+  for (int jj = 0; jj < n; ++jj) {
+    const float4 y = ly[jj];
+    const float d0 = y.x - xr0, d1 = y.y - xr1, d2 = y.z - xr2;
+    float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (EPS) r2 += eps2x2;
+    float w = ls[jj] * __builtin_amdgcn_rcpf(r2);
+    if (jj == self) w = 0.0f;
+    T0 = __builtin_fmaf(w, d0, T0);
+    T1 = __builtin_fmaf(w, d1, T1);
+    T2 = __builtin_fmaf(w, d2, T2);
+  }
+}
+
+// B operands of the distance product for this wave's two 32-target blocks:
+// the lane's target in tile coordinates (offset e from the centre, scale
+// inv), split into f16 hi/lo, gathered so that lane l holds K half h of
+// target column col of each block.
+template <bool EPS>
+__device__ __forceinline__ void mfma16_targets(double e0, double e1, double e2,
+                                               double inv, float eps2x2,
+                                               int col, int h, mh8* B) {
+  const float v0 = static_cast<float>(e0 * inv), v1 = static_cast<float>(e1 * inv),
+              v2 = static_cast<float>(e2 * inv);
+  float p = v0 * v0 + v1 * v1 + v2 * v2;
+  if (EPS) p += eps2x2 * static_cast<float>(inv * inv);
+  _Float16 a0, b0, a1, b1, a2, b2, ph, pl, pll;
+  split_h(-2.0f * v0, a0, b0);
+  split_h(-2.0f * v1, a1, b1);
+  split_h(-2.0f * v2, a2, b2);
+  split_h(p, ph, pl);
+  pll = static_cast<_Float16>(p - static_cast<float>(ph) - static_cast<float>(pl));
+  const _Float16 one = static_cast<_Float16>(1.0f), zero = static_cast<_Float16>(0.0f);
+  // K 0..7: -2vh0 -2vh1 -2vh2 -2vh0 -2vh1 -2vh2 -2vl0 -2vl1
+  // K 8..15: -2vl2 ph pl pll 1 1 0 0
+  const uint32_t own[8] = {pack_h2(a0, a1), pack_h2(a2, a0), pack_h2(a1, a2),
+                           pack_h2(b0, b1), pack_h2(b2, ph), pack_h2(pl, pll),
+                           pack_h2(one, one), pack_h2(zero, zero)};
+#pragma unroll
+  for (int tb = 0; tb < 2; ++tb) {
+    const int from = 32 * tb + col;
+    uint32_t g[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      g[e] = static_cast<uint32_t>(__shfl(static_cast<int>(own[e]), from, 64));
+    const uint4 sel = h ? make_uint4(g[4], g[5], g[6], g[7])
+                        : make_uint4(g[0], g[1], g[2], g[3]);
+    B[tb] = __builtin_bit_cast(mh8, sel);
+  }
+}
+
+// One 32-source chunk: the distance products of both target blocks, their
+// reciprocals and the weight products into D2.  la / lt address the chunk's
+// sources (32 sources: 64 uint4 of A, 32 float4 of T).
+template <int NACC>
+__device__ __forceinline__ void mfma16_chunk(const uint4* la, const float4* lt,
+                                             int col, int h, int c4,
+                                             const mh8* B, mf4 (*D2)[NACC]) {
+  const mh8 A = __builtin_bit_cast(mh8, la[col * 2 + h]);
+  const float4* tp = lt + (h * 4 + c4) * 4;
+  const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+  const float tv[16] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w,
+                        t2.x, t2.y, t2.z, t2.w, t3.x, t3.y, t3.z, t3.w};
+#pragma unroll
+  for (int tb = 0; tb < 2; ++tb) {
+    const mf16 D = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B[tb], mf16{}, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      D2[tb][v % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+          tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb][v % NACC], 0, 0, 0);
+  }
+}
+
+// The tile's (sum w y', sum w) for this lane's target from D2: the two
+// source halves (lanes l, l ^ 32) of each column; lane l keeps column block
+// h, which holds its own target 32 h + col = l.
+template <int NACC>
+__device__ __forceinline__ void mfma16_reduce(mf4 (*D2)[NACC], int h, float* o) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    float v0s = D2[0][0][a], v1s = D2[1][0][a];
+#pragma unroll
+    for (int q = 1; q < NACC; ++q) {
+      v0s += D2[0][q][a];
+      v1s += D2[1][q][a];
+    }
+    v0s += __shfl_xor(v0s, 32, 64);
+    v1s += __shfl_xor(v1s, 32, 64);
+    o[a] = h ? v1s : v0s;
+  }
+}
+
 // NACC: independent weight-product accumulators per target block (1 or 2;
 // two shorten the dependent MFMA chain).
 template <bool EPS, int NACC = 1>
@@ -817,6 +916,7 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;    // K half of the operands, source half of D
   const int col = lane & 31;  // target column
+  const int c4 = lane & 3;
   double S0 = 0.0, S1 = 0.0, S2 = 0.0;
   for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
     const int t = j0 / kPackTile;
@@ -835,95 +935,27 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
     const bool near =
         e0 * e0 + e1 * e1 + e2 * e2 <= kMfmaNear * kMfmaNear * c.w;
     if (__ballot(near)) {
-      const float xr0 = static_cast<float>(e0), xr1 = static_cast<float>(e1),
-                  xr2 = static_cast<float>(e2);
-      const bool diag = static_cast<unsigned>(self[0] - j0) <
-                        static_cast<unsigned>(kPackTile);
       float T0 = 0.0f, T1 = 0.0f, T2 = 0.0f;
-      for (int jj = 0; jj < kPackTile; ++jj) {
-        const float4 y = ly[jj];
-        const float d0 = y.x - xr0, d1 = y.y - xr1, d2 = y.z - xr2;
-        float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        if (EPS) r2 += eps2x2;
-        float w = ls[jj] * __builtin_amdgcn_rcpf(r2);
-        if (diag && j0 + jj == self[0]) w = 0.0f;
-        T0 = __builtin_fmaf(w, d0, T0);
-        T1 = __builtin_fmaf(w, d1, T1);
-        T2 = __builtin_fmaf(w, d2, T2);
-      }
+      mfma16_near<EPS>(ly, ls, kPackTile, static_cast<float>(e0),
+                       static_cast<float>(e1), static_cast<float>(e2), eps2x2,
+                       self[0] - j0, T0, T1, T2);
       S0 += static_cast<double>(T0);
       S1 += static_cast<double>(T1);
       S2 += static_cast<double>(T2);
       continue;
     }
-    // this lane's target in tile coordinates, split
-    const double inv = src.is[t];
-    const float v0 = static_cast<float>(e0 * inv), v1 = static_cast<float>(e1 * inv),
-                v2 = static_cast<float>(e2 * inv);
-    float p = v0 * v0 + v1 * v1 + v2 * v2;
-    if (EPS) p += eps2x2 * static_cast<float>(inv * inv);
-    _Float16 a0, b0, a1, b1, a2, b2, ph, pl, pll;
-    split_h(-2.0f * v0, a0, b0);
-    split_h(-2.0f * v1, a1, b1);
-    split_h(-2.0f * v2, a2, b2);
-    split_h(p, ph, pl);
-    pll = static_cast<_Float16>(p - static_cast<float>(ph) - static_cast<float>(pl));
-    const _Float16 one = static_cast<_Float16>(1.0f), zero = static_cast<_Float16>(0.0f);
-    // K 0..7: -2vh0 -2vh1 -2vh2 -2vh0 -2vh1 -2vh2 -2vl0 -2vl1
-    // K 8..15: -2vl2 ph pl pll 1 1 0 0
-    const uint32_t own[8] = {pack_h2(a0, a1), pack_h2(a2, a0), pack_h2(a1, a2),
-                             pack_h2(b0, b1), pack_h2(b2, ph), pack_h2(pl, pll),
-                             pack_h2(one, one), pack_h2(zero, zero)};
     mh8 B[2];
-#pragma unroll
-    for (int tb = 0; tb < 2; ++tb) {
-      const int from = 32 * tb + col;
-      uint32_t g[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        g[e] = static_cast<uint32_t>(__shfl(static_cast<int>(own[e]), from, 64));
-      const uint4 sel = h ? make_uint4(g[4], g[5], g[6], g[7])
-                          : make_uint4(g[0], g[1], g[2], g[3]);
-      B[tb] = __builtin_bit_cast(mh8, sel);
-    }
+    mfma16_targets<EPS>(e0, e1, e2, src.is[t], eps2x2, col, h, B);
     mf4 D2[2][NACC];
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int q = 0; q < NACC; ++q) D2[tb][q] = mf4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int c4 = lane & 3;
 #pragma unroll 1
-    for (int ch = 0; ch < kPackTile / 32; ++ch) {
-      const mh8 A = __builtin_bit_cast(mh8, la[(32 * ch + col) * 2 + h]);
-      const float4* tp = lt + ((ch * 2 + h) * 4 + c4) * 4;
-      const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
-      const float tv[16] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w,
-                            t2.x, t2.y, t2.z, t2.w, t3.x, t3.y, t3.z, t3.w};
-#pragma unroll
-      for (int tb = 0; tb < 2; ++tb) {
-        const mf16 D = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-            A, B[tb], mf16{}, 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-          D2[tb][v % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(
-              tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb][v % NACC], 0, 0, 0);
-      }
-    }
-    // the two source halves (lanes l, l ^ 32) of each column; lane l keeps
-    // column block h, which holds its own target 32 h + col = l
+    for (int ch = 0; ch < kPackTile / 32; ++ch)
+      mfma16_chunk<NACC>(la + 64 * ch, lt + 32 * ch, col, h, c4, B, D2);
     float o[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      float v0s = D2[0][0][a], v1s = D2[1][0][a];
-#pragma unroll
-      for (int q = 1; q < NACC; ++q) {
-        v0s += D2[0][q][a];
-        v1s += D2[1][q][a];
-      }
-      v0s += __shfl_xor(v0s, 32, 64);
-      v1s += __shfl_xor(v1s, 32, 64);
-      o[a] = h ? v1s : v0s;
-    }
+    mfma16_reduce<NACC>(D2, h, o);
     const double W = static_cast<double>(o[3]);
     S0 += static_cast<double>(o[0]) + c.x * W;
     S1 += static_cast<double>(o[1]) + c.y * W;
