@@ -132,22 +132,39 @@ __device__ __forceinline__ double one_minus_dot(const double* x,
   return 1.0 - t;
 }
 
-template <bool EPS, int NR, bool FUSE = false>
+// S += (s / d) y for one source from its reciprocal distance inv_d = 1 / d.
+// SY: the source's s y is packed (sy), so that the weight needs no multiply;
+// otherwise w = s inv_d scales y.
+template <bool SY>
+__device__ __forceinline__ void acc_w(double* S, double inv_d, const double4& y,
+                                      const double4& sy) {
+  if (SY) {
+    S[0] = __builtin_fma(inv_d, sy.x, S[0]);
+    S[1] = __builtin_fma(inv_d, sy.y, S[1]);
+    S[2] = __builtin_fma(inv_d, sy.z, S[2]);
+  } else {
+    const double w = y.w * inv_d;
+    S[0] = __builtin_fma(w, y.x, S[0]);
+    S[1] = __builtin_fma(w, y.y, S[1]);
+    S[2] = __builtin_fma(w, y.z, S[2]);
+  }
+}
+
+template <bool EPS, int NR, bool FUSE = false, bool SY = false>
 __device__ __forceinline__ void f64_one(const double* x, const double4& y,
-                                        double eps2, double* S) {
+                                        const double4& sy, double eps2,
+                                        double* S) {
   double d = one_minus_dot<FUSE>(x, y);
   if (EPS) d += eps2;
-  const double w = y.w * rcp_nr<NR>(d);
-  S[0] = __builtin_fma(w, y.x, S[0]);
-  S[1] = __builtin_fma(w, y.y, S[1]);
-  S[2] = __builtin_fma(w, y.z, S[2]);
+  acc_w<SY>(S, rcp_nr<NR>(d), y, sy);
 }
 
 // Two sources sharing one reciprocal: r = 1/(da db), 1/da = db r,
 // 1/db = da r.  Terms are accumulated in source order.
-template <bool EPS, int NR, bool FUSE = false>
+template <bool EPS, int NR, bool FUSE = false, bool SY = false>
 __device__ __forceinline__ void f64_two(const double* x, const double4& ya,
-                                        const double4& yb, double eps2,
+                                        const double4& yb, const double4& sya,
+                                        const double4& syb, double eps2,
                                         double* S) {
   double da = one_minus_dot<FUSE>(x, ya), db = one_minus_dot<FUSE>(x, yb);
   if (EPS) {
@@ -155,22 +172,18 @@ __device__ __forceinline__ void f64_two(const double* x, const double4& ya,
     db += eps2;
   }
   const double r = rcp_nr<NR>(da * db);
-  const double wa = ya.w * (db * r);
-  const double wb = yb.w * (da * r);
-  S[0] = __builtin_fma(wa, ya.x, S[0]);
-  S[1] = __builtin_fma(wa, ya.y, S[1]);
-  S[2] = __builtin_fma(wa, ya.z, S[2]);
-  S[0] = __builtin_fma(wb, yb.x, S[0]);
-  S[1] = __builtin_fma(wb, yb.y, S[1]);
-  S[2] = __builtin_fma(wb, yb.z, S[2]);
+  acc_w<SY>(S, db * r, ya, sya);
+  acc_w<SY>(S, da * r, yb, syb);
 }
 
 // Four sources sharing one reciprocal: r = 1/(da db dc dd),
 // 1/da = db (r dc dd), 1/db = da (r dc dd), and so on; terms in source order.
-template <bool EPS, int NR, bool FUSE>
+template <bool EPS, int NR, bool FUSE, bool SY = false>
 __device__ __forceinline__ void f64_four(const double* x, const double4& ya,
                                          const double4& yb, const double4& yc,
-                                         const double4& yd, double eps2,
+                                         const double4& yd, const double4& sya,
+                                         const double4& syb, const double4& syc,
+                                         const double4& syd, double eps2,
                                          double* S) {
   double da = one_minus_dot<FUSE>(x, ya), db = one_minus_dot<FUSE>(x, yb);
   double dc = one_minus_dot<FUSE>(x, yc), dd = one_minus_dot<FUSE>(x, yd);
@@ -183,22 +196,10 @@ __device__ __forceinline__ void f64_four(const double* x, const double4& ya,
   const double pab = da * db, pcd = dc * dd;
   const double r = rcp_nr<NR>(pab * pcd);
   const double rab = r * pcd, rcd = r * pab;
-  const double wa = ya.w * (db * rab);
-  const double wb = yb.w * (da * rab);
-  const double wc = yc.w * (dd * rcd);
-  const double wd = yd.w * (dc * rcd);
-  S[0] = __builtin_fma(wa, ya.x, S[0]);
-  S[1] = __builtin_fma(wa, ya.y, S[1]);
-  S[2] = __builtin_fma(wa, ya.z, S[2]);
-  S[0] = __builtin_fma(wb, yb.x, S[0]);
-  S[1] = __builtin_fma(wb, yb.y, S[1]);
-  S[2] = __builtin_fma(wb, yb.z, S[2]);
-  S[0] = __builtin_fma(wc, yc.x, S[0]);
-  S[1] = __builtin_fma(wc, yc.y, S[1]);
-  S[2] = __builtin_fma(wc, yc.z, S[2]);
-  S[0] = __builtin_fma(wd, yd.x, S[0]);
-  S[1] = __builtin_fma(wd, yd.y, S[1]);
-  S[2] = __builtin_fma(wd, yd.z, S[2]);
+  acc_w<SY>(S, db * rab, ya, sya);
+  acc_w<SY>(S, da * rab, yb, syb);
+  acc_w<SY>(S, dd * rcd, yc, syc);
+  acc_w<SY>(S, dc * rcd, yd, syd);
 }
 
 // Mixed: d = y' - x' (tile-relative FP32), r2 = |d|^2 + 2 eps^2,
@@ -289,16 +290,28 @@ __device__ __forceinline__ void store_cross(const SegArgs& sg,
   }
 }
 
+// The packed s y of source j (SY), or any source record the caller passes
+// where acc_w does not read it.
+template <bool SY>
+__device__ __forceinline__ const double4& sy_of(const double4* tsy, int j,
+                                                const double4& y) {
+  if constexpr (SY) return tsy[j];
+  else return y;
+}
+
 // ---------------------------------------------------------- FP64 block ---
 // Block `blk` of segment `sg` against nsrc packed FP64 sources.  Every
 // thread of the block must call it (it synchronises); `tile` is the
-// block's LDS staging array of TILE sources.  PAIR shares one reciprocal
-// between two sources.
+// block's LDS staging array of TILE sources (and `tile_sy` of their s y
+// when SY).  PAIR shares one reciprocal between two sources, QUAD between
+// four.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false, bool QUAD = false>
+          bool FUSE = false, bool QUAD = false, bool SY = false>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
+                                          const double4* __restrict__ src_sy,
                                           int nsrc, const SegArgs& sg, int blk,
-                                          double eps2, double4* tile) {
+                                          double eps2, double4* tile,
+                                          double4* tile_sy) {
   static_assert(TILE % 8 == 0, "tile must be a multiple of the unroll");
   double x[TPL][3], S[TPL][3];
   int self[TPL];
@@ -307,15 +320,17 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
   if (!load_targets<BLOCK, TPL>(sg, blk, x, self, live, tgt)) return;
 #pragma unroll
   for (int k = 0; k < TPL; ++k) S[k][0] = S[k][1] = S[k][2] = 0.0;
-
   for (int j0 = 0; j0 < nsrc; j0 += TILE) {
     __syncthreads();
     for (int t = threadIdx.x; t < TILE; t += BLOCK) {
       const int j = j0 + t;
       // padding: zero strength at the origin -> 1 - x.y = 1, w = 0
       tile[t] = j < nsrc ? src[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+      if (SY)
+        tile_sy[t] = j < nsrc ? src_sy[j] : make_double4(0.0, 0.0, 0.0, 0.0);
     }
     __syncthreads();
+    const double4* tsy = SY ? tile_sy : tile;  // unread without SY
     bool diag = false;
 #pragma unroll
     for (int k = 0; k < TPL; ++k)
@@ -324,41 +339,49 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
       // tile holds some target's own source: per-pair exclusion
       for (int jj = 0; jj < TILE; ++jj) {
         const double4 y = tile[jj];
+        const double4& sy = sy_of<SY>(tsy, jj, y);
 #pragma unroll
         for (int k = 0; k < TPL; ++k) {
           const double t = x[k][0] * y.x + x[k][1] * y.y + x[k][2] * y.z;
           double d = 1.0 - t;
           if (EPS) d += eps2;
-          double w = y.w * rcp_nr<2>(d);
-          w = (j0 + jj == self[k]) ? 0.0 : w;
-          S[k][0] = __builtin_fma(w, y.x, S[k][0]);
-          S[k][1] = __builtin_fma(w, y.y, S[k][1]);
-          S[k][2] = __builtin_fma(w, y.z, S[k][2]);
+          double r = rcp_nr<2>(d);
+          r = (j0 + jj == self[k]) ? 0.0 : r;
+          acc_w<SY>(S[k], r, y, sy);
         }
       }
     } else if (QUAD) {
       for (int jj = 0; jj < TILE; jj += 4) {
         const double4 ya = tile[jj], yb = tile[jj + 1];
         const double4 yc = tile[jj + 2], yd = tile[jj + 3];
+        const double4& sya = sy_of<SY>(tsy, jj, ya);
+        const double4& syb = sy_of<SY>(tsy, jj + 1, yb);
+        const double4& syc = sy_of<SY>(tsy, jj + 2, yc);
+        const double4& syd = sy_of<SY>(tsy, jj + 3, yd);
 #pragma unroll
         for (int k = 0; k < TPL; ++k)
-          f64_four<EPS, NR, FUSE>(x[k], ya, yb, yc, yd, eps2, S[k]);
+          f64_four<EPS, NR, FUSE, SY>(x[k], ya, yb, yc, yd, sya, syb, syc, syd,
+                                      eps2, S[k]);
       }
     } else if (PAIR) {
 #pragma unroll 2
       for (int jj = 0; jj < TILE; jj += 2) {
         const double4 ya = tile[jj];
         const double4 yb = tile[jj + 1];
+        const double4& sya = sy_of<SY>(tsy, jj, ya);
+        const double4& syb = sy_of<SY>(tsy, jj + 1, yb);
 #pragma unroll
         for (int k = 0; k < TPL; ++k)
-          f64_two<EPS, NR, FUSE>(x[k], ya, yb, eps2, S[k]);
+          f64_two<EPS, NR, FUSE, SY>(x[k], ya, yb, sya, syb, eps2, S[k]);
       }
     } else {
 #pragma unroll 4
       for (int jj = 0; jj < TILE; ++jj) {
         const double4 y = tile[jj];
+        const double4& sy = sy_of<SY>(tsy, jj, y);
 #pragma unroll
-        for (int k = 0; k < TPL; ++k) f64_one<EPS, NR, FUSE>(x[k], y, eps2, S[k]);
+        for (int k = 0; k < TPL; ++k)
+          f64_one<EPS, NR, FUSE, SY>(x[k], y, sy, eps2, S[k]);
       }
     }
   }

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void ens_pack_f64_kernel(
     const double* __restrict__ sx, int64_t es, int64_t cs,
     const double* __restrict__ zeta, const double* __restrict__ area,
     const int32_t* __restrict__ inv, const EnsMember* __restrict__ mem,
-    double4* __restrict__ out) {
+    double4* __restrict__ out, double4* __restrict__ out_sy) {
   const EnsMember M = mem[blockIdx.y];
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int padded = (M.cnt + kPackTile - 1) / kPackTile * kPackTile;
@@ -50,6 +50,8 @@ __global__ __launch_bounds__(256) void ens_pack_f64_kernel(
     v = make_double4(p[0], p[cs], p[2 * cs], -zeta[i] * area[i] * kInv4Pi);
   }
   out[M.P + c] = v;
+  // {s y, 0} for the kernels that take the weight from it
+  if (out_sy) out_sy[M.P + c] = make_double4(v.w * v.x, v.w * v.y, v.w * v.z, 0.0);
 }
 
 // Targets of work item segment `seg` of member M, as a SegArgs whose self
@@ -104,11 +106,13 @@ __device__ __forceinline__ int ens_next(const EnsArgs& A, int* slot) {
 }
 
 template <int BLOCK, int TPL, int NR, bool PAIR, bool FUSE = false,
-          bool QUAD = false>
+          bool QUAD = false, bool SY = false>
 __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
-    const double4* __restrict__ src, EnsArgs A) {
+    const double4* __restrict__ src, const double4* __restrict__ src_sy,
+    EnsArgs A) {
   constexpr int TILE = 256;
   __shared__ double4 tile[TILE];
+  __shared__ double4 tile_sy[SY ? TILE : 1];
   __shared__ int slot;
   for (;;) {
     const int it = ens_next(A, &slot);
@@ -116,8 +120,9 @@ __global__ __launch_bounds__(BLOCK) void ens_f64_kernel(
     const int4 w = A.items[it];
     const EnsMember M = A.mem[w.x];
     const SegArgs sg = ens_segment(A, M, w.y);
-    f64_block<BLOCK, TPL, TILE, false, NR, PAIR, FUSE, QUAD>(src + M.P, M.cnt, sg, w.z,
-                                                 0.0, tile);
+    f64_block<BLOCK, TPL, TILE, false, NR, PAIR, FUSE, QUAD, SY>(
+        src + M.P, SY ? src_sy + M.P : nullptr, M.cnt, sg, w.z, 0.0, tile,
+        tile_sy);
   }
 }
 
@@ -182,7 +187,7 @@ __global__ __launch_bounds__(256, MINW) void ens_mfma16_kernel(Mfma16Src src,
 
 // ------------------------------------------------------ kernel registry ---
 // Every variant has a one-problem twin of the same shape in bve_sums.hip.
-using EnsF64Fn = void (*)(const double4*, EnsArgs);
+using EnsF64Fn = void (*)(const double4*, const double4*, EnsArgs);
 using EnsMixedFn = void (*)(const float4*, const double4*, EnsArgs);
 using EnsMfmaFn = void (*)(MfmaSrc, EnsArgs);
 using EnsMfma16Fn = void (*)(Mfma16Src, EnsArgs);
@@ -221,6 +226,8 @@ const EnsVariant kEnsF64[] = {
      "persistent_block256_tpl4_nr1_pairtrue_fused", nullptr},  // 6
     {ens_f64_kernel<256, 4, 1, true, true, true>, nullptr, 256, 4,
      "persistent_block256_tpl4_nr1_quad_fused", nullptr},      // 7
+    {ens_f64_kernel<256, 4, 1, true, true, true, true>, nullptr, 256, 4,
+     "persistent_block256_tpl4_nr1_quad_fused_sy", nullptr},   // 8: s y packed
 };
 const EnsVariant kEnsMixed[] = {
     LPM_ENS_MIXED(256, 2, false),  // 0: fixed default
@@ -272,6 +279,7 @@ const EnsVariant& pick_ens_variant(lpm_ctx* ctx, const EnsembleDesc& d,
 void ensure_sources(lpm_ctx* ctx, const EnsembleDesc& d, int precision) {
   const int64_t cap = std::max<int64_t>(d.total_padded_upper, kPackTile);
   ctx->packed64.ensure(sizeof(double4) * cap);
+  if (precision == LPM_FP64) ctx->packed64sy.ensure(sizeof(double4) * cap);
   if (precision == LPM_MIXED) {
     ctx->packed32.ensure(sizeof(float4) * cap);
     ctx->centers.ensure(sizeof(double4) * (cap / kPackTile));
@@ -308,10 +316,13 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
       ensemble_layout(ctx, d, map, f.face_mask, v.block * v.tpl, s);
 
   double4* p64 = ctx->packed64.as<double4>();
+  double4* psy =
+      precision == LPM_FP64 ? ctx->packed64sy.as<double4>() : nullptr;
   const dim3 pgrid(static_cast<unsigned>((d.max_member_faces + 255) / 256),
                    static_cast<unsigned>(M));
   ens_pack_f64_kernel<<<pgrid, 256, 0, s>>>(fx.base, fx.es, fx.cs, fzeta,
-                                             f.face_area, map.inv, L.mem, p64);
+                                             f.face_area, map.inv, L.mem, p64,
+                                             psy);
   LPM_HIP_CHECK(hipGetLastError());
 
   EnsArgs A{};
@@ -379,7 +390,8 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
     hipLaunchKernelGGL(v.mixed, dim3(grid), dim3(v.block), 0, s, p32, cen, A);
   else
     hipLaunchKernelGGL(v.f64, dim3(grid), dim3(v.block), 0, s,
-                       static_cast<const double4*>(p64), A);
+                       static_cast<const double4*>(p64),
+                       static_cast<const double4*>(psy), A);
   LPM_HIP_CHECK(hipGetLastError());
 }
 

@@ -117,33 +117,40 @@ __global__ void set_count_kernel(int32_t* __restrict__ count, int32_t value) {
   *count = value;
 }
 
-// Gathers the packed sources in packed order (coalesced writes).
+// Gathers the packed sources in packed order (coalesced writes); out_sy
+// (optional) receives {s y, 0} for the kernels that take the weight from it.
 __global__ __launch_bounds__(256) void pack_f64_kernel(
     const double* __restrict__ sx, int64_t es, int64_t cs,
     const double* __restrict__ zeta, const double* __restrict__ area,
     const int32_t* __restrict__ inv, const int32_t* __restrict__ count,
-    double4* __restrict__ out) {
+    double4* __restrict__ out, double4* __restrict__ out_sy) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= *count) return;
   const int64_t i = inv ? inv[r] : r;
   const double* p = sx + i * es;
-  out[r] = make_double4(p[0], p[cs], p[2 * cs], -zeta[i] * area[i] * kInv4Pi);
+  const double y0 = p[0], y1 = p[cs], y2 = p[2 * cs];
+  const double sv = -zeta[i] * area[i] * kInv4Pi;
+  out[r] = make_double4(y0, y1, y2, sv);
+  if (out_sy) out_sy[r] = make_double4(sv * y0, sv * y1, sv * y2, 0.0);
 }
 
 // ------------------------------------------------------- velocity sums ---
 // One launch covers two target segments (vertices, then faces): blocks
 // [0, s0.nblocks) take s0, the rest s1.
+// SY: the weights come from the packed s y (src_sy), one multiply less per
+// pair; otherwise src_sy is not read.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false, bool QUAD = false>
+          bool FUSE = false, bool QUAD = false, bool SY = false>
 __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
-    const double4* __restrict__ src, const int32_t* __restrict__ count,
-    SegArgs s0, SegArgs s1, double eps2) {
+    const double4* __restrict__ src, const double4* __restrict__ src_sy,
+    const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, double eps2) {
   __shared__ double4 tile[TILE];
+  __shared__ double4 tile_sy[SY ? TILE : 1];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD>(src, *count, sg, blk,
-                                                        eps2, tile);
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD, SY>(
+      src, src_sy, *count, sg, blk, eps2, tile, tile_sy);
 }
 
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
@@ -281,8 +288,8 @@ __global__ __launch_bounds__(256) void finite_kernel(
 __global__ void zero_int_kernel(int32_t* p) { *p = 0; }
 
 // ------------------------------------------------------ kernel registry ---
-using F64Kernel = void (*)(const double4*, const int32_t*, SegArgs, SegArgs,
-                           double);
+using F64Kernel = void (*)(const double4*, const double4*, const int32_t*,
+                          SegArgs, SegArgs, double);
 using MixedKernel = void (*)(const float4*, const double4*, const int32_t*,
                              SegArgs, SegArgs, float);
 using MfmaKernel = void (*)(MfmaSrc, const int32_t*, SegArgs, SegArgs, float);
@@ -293,6 +300,7 @@ struct F64Variant {
   F64Kernel fn, fn_eps;
   int block, tpl;
   const char* name;
+  bool sy = false;  // reads the packed s y
 };
 struct MixedVariant {
   MixedKernel fn, fn_eps;  // vector-ALU kernels (or null)
@@ -314,6 +322,10 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true>,          \
    bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true>, B, T,     \
    "block" #B "_tpl" #T "_nr" #NR "_quad_fused"}
+#define LPM_F64_QUAD_SY(B, T, NR)                                        \
+  {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true, true>,    \
+   bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true, true>, B, T, \
+   "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy", true}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -333,6 +345,8 @@ const F64Variant kF64Variants[] = {
     LPM_F64_FUSED(256, 1, 1),           // 10
     LPM_F64_QUAD(256, 2, 1),            // 11: one reciprocal per 4 pairs
     LPM_F64_QUAD(256, 4, 1),            // 12
+    LPM_F64_QUAD_SY(256, 2, 1),         // 13: weights from the packed s y
+    LPM_F64_QUAD_SY(256, 4, 1),         // 14
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
@@ -477,12 +491,17 @@ PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
   const int64_t cap = std::max<int64_t>(n, 1);
   double4* p64 =
       static_cast<double4*>(ctx->packed64.ensure(sizeof(double4) * cap));
+  double4* psy =
+      precision == LPM_FP64
+          ? static_cast<double4*>(ctx->packed64sy.ensure(sizeof(double4) * cap))
+          : nullptr;
   if (n > 0) {
     pack_f64_kernel<<<static_cast<int>((n + 255) / 256), 256, 0, s>>>(
-        sx.base, sx.es, sx.cs, zeta, area, map.inv, map.d_count, p64);
+        sx.base, sx.es, sx.cs, zeta, area, map.inv, map.d_count, p64, psy);
     LPM_HIP_CHECK(hipGetLastError());
   }
   ps.data = p64;
+  ps.sy = psy;
   if (precision == LPM_MIXED) {
     const int ntile = static_cast<int>((cap + kPackTile - 1) / kPackTile);
     float4* p32 =
@@ -541,10 +560,11 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
+    LPM_ARG_CHECK(!v.sy || src.sy, "packed s y missing for the FP64 variant");
     TimedLaunch tl(ctx, KID_BVE_VEL_F64, s, pairs);
     hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block), 0, s,
-                       static_cast<const double4*>(src.data), src.count, a0,
-                       a1, eps2);
+                       static_cast<const double4*>(src.data), src.sy, src.count,
+                       a0, a1, eps2);
     LPM_HIP_CHECK(hipGetLastError());
   } else if (precision == LPM_MIXED) {
     const MixedVariant& v =

@@ -101,6 +101,7 @@ struct PackedSources {
   const void* data = nullptr;      // double4 {x,y,z,s} (FP64) or float4
                                    // {x',y',z',2s} tile-relative (mixed)
   const double4* centers = nullptr; // mixed: tile centres (x, y, z, -)
+  const double4* sy = nullptr;     // FP64: {s y, 0} per source
   const int32_t* count = nullptr;  // device: number of packed sources
   int64_t capacity = 0;            // host upper bound (n stored faces)
   // mixed: matrix-core layout (see bve_device.hpp)
@@ -153,6 +154,7 @@ struct lpm_ctx {
   lpm_amd::DevBuf inv;          // int32 per stored face (packed -> face)
   lpm_amd::DevBuf block_counts; // int32 per scan block (+1 total)
   lpm_amd::DevBuf packed64;     // double4 per source
+  lpm_amd::DevBuf packed64sy;   // double4 {s y, 0} per source (FP64)
   lpm_amd::DevBuf packed32;     // float4 per source (mixed)
   lpm_amd::DevBuf centers;      // double4 per tile (mixed)
   lpm_amd::DevBuf mf_y, mf_q, mf_s, mf_t, mf_c;  // matrix-core layout (mixed)
