@@ -30,6 +30,8 @@
 // the pairs.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 
@@ -310,15 +312,33 @@ __device__ __forceinline__ void swe_f64_block(
     const int cnt = min(kSweTile, nsrc - j0);
     const bool diag = static_cast<unsigned>(self - j0) <
                       static_cast<unsigned>(cnt);
-    for (int jj = 0; jj < cnt; ++jj) {
-      const double4 yp = tile[jj].p;
-      const double4 yq = tile[jj].q;
-      const double d0 = yp.x - x[0], d1 = yp.y - x[1], d2 = yp.z - x[2];
-      const double t = x[0] * yp.x + x[1] * yp.y + x[2] * yp.z;
-      const double delta = 1.0 - t;
-      double r = rcp_nr<2>(delta + eps2);
-      if (diag && j0 + jj == self) r = 0.0;
-      swe_pair<double>(acc, d0, d1, d2, delta, r, yp.w, yq.x);
+    if (__ballot(diag)) {
+      // the tile holds some lane's own source: per-pair exclusion
+      for (int jj = 0; jj < cnt; ++jj) {
+        const double4 yp = tile[jj].p;
+        const double4 yq = tile[jj].q;
+        const double d0 = yp.x - x[0], d1 = yp.y - x[1], d2 = yp.z - x[2];
+        const double t = x[0] * yp.x + x[1] * yp.y + x[2] * yp.z;
+        const double delta = 1.0 - t;
+        double r = rcp_nr<2>(delta + eps2);
+        if (j0 + jj == self) r = 0.0;
+        swe_pair<double>(acc, d0, d1, d2, delta, r, yp.w, yq.x);
+      }
+    } else {
+      // two sources share one reciprocal, 1/D_a = D_b / (D_a D_b); the
+      // zero padding past cnt contributes nothing
+      for (int jj = 0; jj < cnt; jj += 2) {
+        const double4 ya = tile[jj].p, qa = tile[jj].q;
+        const double4 yb = tile[jj + 1].p, qb = tile[jj + 1].q;
+        const double a0 = ya.x - x[0], a1 = ya.y - x[1], a2 = ya.z - x[2];
+        const double b0 = yb.x - x[0], b1 = yb.y - x[1], b2 = yb.z - x[2];
+        const double da = 1.0 - (x[0] * ya.x + x[1] * ya.y + x[2] * ya.z);
+        const double db = 1.0 - (x[0] * yb.x + x[1] * yb.y + x[2] * yb.z);
+        const double Da = da + eps2, Db = db + eps2;
+        const double r = rcp_nr<2>(Da * Db);
+        swe_pair<double>(acc, a0, a1, a2, da, Db * r, ya.w, qa.x);
+        swe_pair<double>(acc, b0, b1, b2, db, Da * r, yb.w, qb.x);
+      }
     }
     if (do_pse) {
       // wave-uniform test: is any target of the wave near this tile?
@@ -429,16 +449,23 @@ __device__ __forceinline__ void swe_mixed_block(
                       static_cast<unsigned>(cnt);
     SweAcc<float> p;
     p.zero();
-    for (int jj = 0; jj < cnt; ++jj) {
-      const float4 yp = tile[jj].p;
-      const float4 yq = tile[jj].q;
-      const float d0 = yp.x - xr[0], d1 = yp.y - xr[1], d2 = yp.z - xr[2];
-      const float rho2 = d0 * d0 + d1 * d1 + d2 * d2;
-      const float delta = 0.5f * rho2;
-      float r = __builtin_amdgcn_rcpf(delta + e2f);
-      if (diag && j0 + jj == self) r = 0.0f;
-      swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
-    }
+    // the per-pair self test only in tiles holding some lane's own source
+    auto pairs = [&](auto check) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float4 yp = tile[jj].p;
+        const float4 yq = tile[jj].q;
+        const float d0 = yp.x - xr[0], d1 = yp.y - xr[1], d2 = yp.z - xr[2];
+        const float rho2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const float delta = 0.5f * rho2;
+        float r = __builtin_amdgcn_rcpf(delta + e2f);
+        if (decltype(check)::value && j0 + jj == self) r = 0.0f;
+        swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
+      }
+    };
+    if (__ballot(diag))
+      pairs(std::true_type{});
+    else
+      pairs(std::false_type{});
     // carry the tile's FP32 partials into the FP64 accumulators
     acc.A0 += p.A0;
     acc.Bs += p.Bs;
