@@ -304,7 +304,9 @@ __device__ __forceinline__ const double4& sy_of(const double4* tsy, int j,
 // thread of the block must call it (it synchronises); `tile` is the
 // block's LDS staging array of TILE sources (and `tile_sy` of their s y
 // when SY).  PAIR shares one reciprocal between two sources, QUAD between
-// four.
+// four.  Each tile is summed into its own partial T, then added to S: two
+// levels of summation keep the rounding of S (whose component along x
+// cancels in x cross S) from growing with every source.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
           bool FUSE = false, bool QUAD = false, bool SY = false>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
@@ -321,6 +323,9 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
 #pragma unroll
   for (int k = 0; k < TPL; ++k) S[k][0] = S[k][1] = S[k][2] = 0.0;
   for (int j0 = 0; j0 < nsrc; j0 += TILE) {
+    double T[TPL][3];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0;
     __syncthreads();
     for (int t = threadIdx.x; t < TILE; t += BLOCK) {
       const int j = j0 + t;
@@ -347,7 +352,7 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
           if (EPS) d += eps2;
           double r = rcp_nr<2>(d);
           r = (j0 + jj == self[k]) ? 0.0 : r;
-          acc_w<SY>(S[k], r, y, sy);
+          acc_w<SY>(T[k], r, y, sy);
         }
       }
     } else if (QUAD) {
@@ -361,7 +366,7 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
 #pragma unroll
         for (int k = 0; k < TPL; ++k)
           f64_four<EPS, NR, FUSE, SY>(x[k], ya, yb, yc, yd, sya, syb, syc, syd,
-                                      eps2, S[k]);
+                                      eps2, T[k]);
       }
     } else if (PAIR) {
 #pragma unroll 2
@@ -372,7 +377,7 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
         const double4& syb = sy_of<SY>(tsy, jj + 1, yb);
 #pragma unroll
         for (int k = 0; k < TPL; ++k)
-          f64_two<EPS, NR, FUSE, SY>(x[k], ya, yb, sya, syb, eps2, S[k]);
+          f64_two<EPS, NR, FUSE, SY>(x[k], ya, yb, sya, syb, eps2, T[k]);
       }
     } else {
 #pragma unroll 4
@@ -381,8 +386,14 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
         const double4& sy = sy_of<SY>(tsy, jj, y);
 #pragma unroll
         for (int k = 0; k < TPL; ++k)
-          f64_one<EPS, NR, FUSE, SY>(x[k], y, sy, eps2, S[k]);
+          f64_one<EPS, NR, FUSE, SY>(x[k], y, sy, eps2, T[k]);
       }
+    }
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      S[k][0] += T[k][0];
+      S[k][1] += T[k][1];
+      S[k][2] += T[k][2];
     }
   }
   store_cross<TPL>(sg, x, S, live, tgt);
