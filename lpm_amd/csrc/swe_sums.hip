@@ -44,6 +44,9 @@ namespace {
 constexpr double kInv4Pi = 0.079577471545947667884441881686257181;
 constexpr double k4OverPi = 1.2732395447351626861510701069801148;
 constexpr int kSweTile = 256;
+// mixed kernel: source tiles this many bounding radii from a target of the
+// wave (centre = the tile's first source) take the FP64 pair loop
+constexpr double kSweNearFp64 = 2.0;
 constexpr double kPseCut = 50.0;  // exp(-50) ~ 2e-22
 
 struct SweSrc {
@@ -267,6 +270,42 @@ __device__ __forceinline__ void swe_finalize(const SweAcc<double>& A, const doub
   *laps = pse_scale * A.L1;
 }
 
+// FP64 pair terms of one LDS tile (cnt sources, zero padding up to an even
+// count) for target x; `self` is the target's own source in tile numbering
+// (outside [0, cnt): none) and any_self says whether some lane of the wave
+// has one here.  Without, two sources share one reciprocal,
+// 1/D_a = D_b / (D_a D_b).
+__device__ __forceinline__ void swe_tile_f64(SweAcc<double>& acc,
+                                             const SweSrc* tile, int cnt,
+                                             const double* x, double eps2,
+                                             int self, bool any_self) {
+  if (any_self) {
+    for (int jj = 0; jj < cnt; ++jj) {
+      const double4 yp = tile[jj].p;
+      const double4 yq = tile[jj].q;
+      const double d0 = yp.x - x[0], d1 = yp.y - x[1], d2 = yp.z - x[2];
+      const double t = x[0] * yp.x + x[1] * yp.y + x[2] * yp.z;
+      const double delta = 1.0 - t;
+      double r = rcp_nr<2>(delta + eps2);
+      if (jj == self) r = 0.0;
+      swe_pair<double>(acc, d0, d1, d2, delta, r, yp.w, yq.x);
+    }
+  } else {
+    for (int jj = 0; jj < cnt; jj += 2) {
+      const double4 ya = tile[jj].p, qa = tile[jj].q;
+      const double4 yb = tile[jj + 1].p, qb = tile[jj + 1].q;
+      const double a0 = ya.x - x[0], a1 = ya.y - x[1], a2 = ya.z - x[2];
+      const double b0 = yb.x - x[0], b1 = yb.y - x[1], b2 = yb.z - x[2];
+      const double da = 1.0 - (x[0] * ya.x + x[1] * ya.y + x[2] * ya.z);
+      const double db = 1.0 - (x[0] * yb.x + x[1] * yb.y + x[2] * yb.z);
+      const double Da = da + eps2, Db = db + eps2;
+      const double r = rcp_nr<2>(Da * Db);
+      swe_pair<double>(acc, a0, a1, a2, da, Db * r, ya.w, qa.x);
+      swe_pair<double>(acc, b0, b1, b2, db, Da * r, yb.w, qb.x);
+    }
+  }
+}
+
 // ----------------------------------------------------------- FP64 kernel --
 // One target per lane; sources staged through LDS.  PSE terms are evaluated
 // only for tiles near some target of the wave.
@@ -312,34 +351,7 @@ __device__ __forceinline__ void swe_f64_block(
     const int cnt = min(kSweTile, nsrc - j0);
     const bool diag = static_cast<unsigned>(self - j0) <
                       static_cast<unsigned>(cnt);
-    if (__ballot(diag)) {
-      // the tile holds some lane's own source: per-pair exclusion
-      for (int jj = 0; jj < cnt; ++jj) {
-        const double4 yp = tile[jj].p;
-        const double4 yq = tile[jj].q;
-        const double d0 = yp.x - x[0], d1 = yp.y - x[1], d2 = yp.z - x[2];
-        const double t = x[0] * yp.x + x[1] * yp.y + x[2] * yp.z;
-        const double delta = 1.0 - t;
-        double r = rcp_nr<2>(delta + eps2);
-        if (j0 + jj == self) r = 0.0;
-        swe_pair<double>(acc, d0, d1, d2, delta, r, yp.w, yq.x);
-      }
-    } else {
-      // two sources share one reciprocal, 1/D_a = D_b / (D_a D_b); the
-      // zero padding past cnt contributes nothing
-      for (int jj = 0; jj < cnt; jj += 2) {
-        const double4 ya = tile[jj].p, qa = tile[jj].q;
-        const double4 yb = tile[jj + 1].p, qb = tile[jj + 1].q;
-        const double a0 = ya.x - x[0], a1 = ya.y - x[1], a2 = ya.z - x[2];
-        const double b0 = yb.x - x[0], b1 = yb.y - x[1], b2 = yb.z - x[2];
-        const double da = 1.0 - (x[0] * ya.x + x[1] * ya.y + x[2] * ya.z);
-        const double db = 1.0 - (x[0] * yb.x + x[1] * yb.y + x[2] * yb.z);
-        const double Da = da + eps2, Db = db + eps2;
-        const double r = rcp_nr<2>(Da * Db);
-        swe_pair<double>(acc, a0, a1, a2, da, Db * r, ya.w, qa.x);
-        swe_pair<double>(acc, b0, b1, b2, db, Da * r, yb.w, qb.x);
-      }
-    }
+    swe_tile_f64(acc, tile, cnt, x, eps2, self - j0, __ballot(diag) != 0);
     if (do_pse) {
       // wave-uniform test: is any target of the wave near this tile?
       const double4 bd = bounds[j0 / kSweTile];
@@ -447,40 +459,49 @@ __device__ __forceinline__ void swe_mixed_block(
     const int cnt = min(kSweTile, nsrc - j0);
     const bool diag = static_cast<unsigned>(self - j0) <
                       static_cast<unsigned>(cnt);
-    SweAcc<float> p;
-    p.zero();
-    // the per-pair self test only in tiles holding some lane's own source
-    auto pairs = [&](auto check) {
-      for (int jj = 0; jj < cnt; ++jj) {
-        const float4 yp = tile[jj].p;
-        const float4 yq = tile[jj].q;
-        const float d0 = yp.x - xr[0], d1 = yp.y - xr[1], d2 = yp.z - xr[2];
-        const float rho2 = d0 * d0 + d1 * d1 + d2 * d2;
-        const float delta = 0.5f * rho2;
-        float r = __builtin_amdgcn_rcpf(delta + e2f);
-        if (decltype(check)::value && j0 + jj == self) r = 0.0f;
-        swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
-      }
-    };
-    if (__ballot(diag))
-      pairs(std::true_type{});
-    else
-      pairs(std::false_type{});
-    // carry the tile's FP32 partials into the FP64 accumulators
-    acc.A0 += p.A0;
-    acc.Bs += p.Bs;
-    acc.Sbd += p.Sbd;
-    acc.Sd2 += p.Sd2;
+    const double g0 = x[0] - bd.x, g1 = x[1] - bd.y, g2 = x[2] - bd.z;
+    // tiles within kSweNearFp64 bounding radii of some target of the wave
+    // hold the largest pair terms of grad u, whose assembly cancels: those
+    // run in FP64 straight into the FP64 accumulators
+    if (__ballot(in && g0 * g0 + g1 * g1 + g2 * g2 <=
+                           kSweNearFp64 * kSweNearFp64 * bd.w)) {
+      swe_tile_f64(acc, tile64, cnt, x, eps2, self - j0, __ballot(diag) != 0);
+    } else {
+      SweAcc<float> p;
+      p.zero();
+      // the per-pair self test only in tiles holding some lane's own source
+      auto pairs = [&](auto check) {
+        for (int jj = 0; jj < cnt; ++jj) {
+          const float4 yp = tile[jj].p;
+          const float4 yq = tile[jj].q;
+          const float d0 = yp.x - xr[0], d1 = yp.y - xr[1], d2 = yp.z - xr[2];
+          const float rho2 = d0 * d0 + d1 * d1 + d2 * d2;
+          const float delta = 0.5f * rho2;
+          float r = __builtin_amdgcn_rcpf(delta + e2f);
+          if (decltype(check)::value && j0 + jj == self) r = 0.0f;
+          swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
+        }
+      };
+      if (__ballot(diag))
+        pairs(std::true_type{});
+      else
+        pairs(std::false_type{});
+      // carry the tile's FP32 partials into the FP64 accumulators
+      acc.A0 += p.A0;
+      acc.Bs += p.Bs;
+      acc.Sbd += p.Sbd;
+      acc.Sd2 += p.Sd2;
 
-    for (int k = 0; k < 3; ++k) {
-      acc.Vd[k] += p.Vd[k];
-      acc.Wd[k] += p.Wd[k];
-      acc.Vs[k] += p.Vs[k];
-      acc.Ws[k] += p.Ws[k];
-    }
-    for (int k = 0; k < 6; ++k) {
-      acc.Yd[k] += p.Yd[k];
-      acc.Ys[k] += p.Ys[k];
+      for (int k = 0; k < 3; ++k) {
+        acc.Vd[k] += p.Vd[k];
+        acc.Wd[k] += p.Wd[k];
+        acc.Vs[k] += p.Vs[k];
+        acc.Ws[k] += p.Ws[k];
+      }
+      for (int k = 0; k < 6; ++k) {
+        acc.Yd[k] += p.Yd[k];
+        acc.Ys[k] += p.Ys[k];
+      }
     }
     if (do_pse) {
       const double e0 = x[0] - bd.x, e1 = x[1] - bd.y, e2 = x[2] - bd.z;

@@ -3,8 +3,9 @@ PSE surface Laplacian) and the sphere SWE RK4 stepper against the FP64
 oracle.
 
 Tolerances: FP64 <= 1e-12 relative L2 (the PSE sum omits terms below
-exp(-50) relative, far under that); mixed precision: velocity <= 1e-6,
-double dot and Laplacian <= 1e-5 relative L2 against the FP64 kernel.
+exp(-50) relative, far under that); mixed precision: velocity, double dot
+and Laplacian <= 1e-6 relative L2 against the FP64 kernel (near tiles and
+the PSE sum run in FP64 in the mixed kernel).
 """
 import numpy as np
 import pytest
@@ -91,8 +92,8 @@ def test_swe_sums_mixed_match_fp64(ctx):
     u32, d32, l32 = (t.cpu().numpy() for t in out["mixed"])
     lf = leaf.numpy()
     assert rel_l2(u32[:, lf], u64[:, lf]) < 1e-6
-    assert rel_l2(d32[lf], d64[lf]) < 1e-5
-    assert rel_l2(l32[lf], l64[lf]) < 1e-5
+    assert rel_l2(d32[lf], d64[lf]) < 1e-6
+    assert rel_l2(l32[lf], l64[lf]) < 1e-6
 
 
 def test_swe_sums_tc2_closed_forms(ctx):
