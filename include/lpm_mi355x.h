@@ -445,6 +445,28 @@ int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
 int lpm_tune_save(lpm_ctx* ctx, const char* path);
 int lpm_tune_load(lpm_ctx* ctx, const char* path);
 
+/* ------------------------------------------------------- diagnostics ---- */
+/* Relative error norms of a device field against its exact values:
+ * replaces the ErrNorms constructors with their error computation and
+ * parallel_reduce (src/lpm_error.hpp:81-131, src/lpm_error_impl.hpp:61-182).
+ * Element i of appx / exact / err has ncomp (1..3) components at
+ * base[i * es + k * cs]; e_i = (mask && mask[i]) ? 0 : appx_i - exact_i,
+ * written to err when err is not NULL (err may be appx).  weight (NULL: 1)
+ * weighs the l1 and l2 sums; masked elements keep their exact magnitude in
+ * the denominators.  norms (host) receives {l1, l2, linf}, each quotient
+ * num * den / (den^2 + eps^2) with eps the FP64 machine epsilon.  Blocking
+ * on the stream, as the reference's reduction to the host is. */
+int lpm_err_norms(lpm_ctx* ctx, const double* appx, int64_t appx_es,
+                  int64_t appx_cs, const double* exact, int64_t exact_es,
+                  int64_t exact_cs, const double* weight, const uint8_t* mask,
+                  int64_t n, int ncomp, double* err, int64_t err_es,
+                  int64_t err_cs, double* norms, void* stream);
+/* u_i . x_i per particle into out (device, n values): SphereTangentFunctor
+ * (src/lpm_sphere_functions.hpp:62-78).  Stream-ordered. */
+int lpm_sphere_tangency(lpm_ctx* ctx, const double* u, int64_t u_es,
+                        int64_t u_cs, const double* x, int64_t x_es,
+                        int64_t x_cs, int64_t n, double* out, void* stream);
+
 /* ------------------------------------------------------------ timing ---- */
 /* HIP events recorded around every direct-sum launch, on the stream the
  * kernel is launched on (disabled inside graph capture).  Counterpart of the

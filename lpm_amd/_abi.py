@@ -186,6 +186,11 @@ _SIGS = {
     "lpm_tune_load": (c_int, [c_vp, ctypes.c_char_p]),
     "lpm_timing_enable": (c_int, [c_vp, c_int]),
     "lpm_timing_query": (c_int, [c_vp, c_int, c_dblp, c_i64p, c_dblp]),
+    "lpm_err_norms": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64,
+                              c_vp, c_vp, c_i64, c_int, c_vp, c_i64, c_i64,
+                              c_dblp, c_vp]),
+    "lpm_sphere_tangency": (c_int, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64,
+                                    c_i64, c_i64, c_vp, c_vp]),
 }
 
 _lib = None

@@ -160,6 +160,7 @@ struct lpm_ctx {
   lpm_amd::DevBuf mf_y, mf_q, mf_s, mf_t, mf_c;  // matrix-core layout (mixed)
   lpm_amd::DevBuf mf_a, mf_t32, mf_is;          // f16-split layout (mixed)
   lpm_amd::DevBuf flag;         // int32 non-finite flag
+  lpm_amd::DevBuf diag;         // error-norm partial sums
   lpm_amd::DevBuf count_ident;  // int32 count when no mask
 
   lpm_amd::DevBuf swe_packed;   // SWE sources (64 B each)

@@ -277,6 +277,47 @@ class Context:
     def timing_enable(self, enable=True):
         self.check(self.L.lpm_timing_enable(self.h, int(enable)))
 
+    # -------------------------------------------------------- diagnostics
+    def err_norms(self, appx, exact, weight=None, mask=None, err=None,
+                  stream=None):
+        """ErrNorms of appx against exact: (l1, l2, linf).  appx / exact /
+        err are (ncomp, n) device tensors (or (n,) for scalars); mask marks
+        elements whose error is taken as zero; err receives appx - exact."""
+        def fld(t):
+            if t.dim() == 1:
+                return t.data_ptr(), t.stride(0), 1, 1, t.shape[0]
+            return t.data_ptr(), t.stride(1), t.stride(0), t.shape[0], t.shape[1]
+        for t in (appx, exact) + ((err,) if err is not None else ()):
+            if t.dtype != torch.float64 or not t.is_cuda:
+                raise TypeError("fields must be float64 device tensors")
+        ap, aes, acs, nc, n = fld(appx)
+        xp, xes, xcs, nc2, n2 = fld(exact)
+        if (nc, n) != (nc2, n2):
+            raise ValueError("appx and exact shapes differ")
+        ep, ees, ecs = (None, 1, 1)
+        if err is not None:
+            ep, ees, ecs, nc3, n3 = fld(err)
+            if (nc3, n3) != (nc, n):
+                raise ValueError("err shape differs")
+        out = (ctypes.c_double * 3)()
+        self.check(self.L.lpm_err_norms(
+            self.h, ap, aes, acs, xp, xes, xcs, _ptr(weight), _ptr(mask), n, nc,
+            ep, ees, ecs, out, stream if stream is not None else current_stream()))
+        return out[0], out[1], out[2]
+
+    def sphere_tangency(self, u, x, out=None, stream=None):
+        """u . x per particle of (3, n) device tensors (SphereTangentFunctor)."""
+        up, ues, ucs, n = crd_args(u)
+        xp, xes, xcs, n2 = crd_args(x)
+        if n != n2:
+            raise ValueError("u and x sizes differ")
+        if out is None:
+            out = torch.zeros(n, dtype=torch.float64, device=u.device)
+        self.check(self.L.lpm_sphere_tangency(
+            self.h, up, ues, ucs, xp, xes, xcs, n, _ptr(out),
+            stream if stream is not None else current_stream()))
+        return out
+
     def timing_query(self, kernel_id):
         ms = ctypes.c_double()
         n = ctypes.c_int64()

@@ -19,6 +19,7 @@
 #include <string>
 
 #include "lpm/bve_sphere.hpp"
+#include "lpm/error.hpp"
 #include "lpm/vtk.hpp"
 
 using namespace Lpm;
@@ -84,6 +85,25 @@ int run(const Args& a) {
     es += ds * ds;
     ns += OMG * z * OMG * z;
   }
+  // the same vertex velocity error with LPM's ErrNorms on the device, and
+  // the largest |u . x| (SphereTangentFunctor)
+  const int64_t nvh = sphere.n_vertices_host();
+  VectorField uex(static_cast<size_t>(nvh));
+  for (Index i = 0; i < sphere.n_vertices_host(); ++i) {
+    uex.host(i, 0) = -OMG * sphere.vert_x.host(i, 1);
+    uex.host(i, 1) = OMG * sphere.vert_x.host(i, 0);
+    uex.host(i, 2) = 0;
+  }
+  uex.update_device();
+  const Context& ctx = *Context::default_context();
+  const ErrNorms vel_norms(ctx, nullptr, sphere.velocity_verts, uex, nullptr,
+                           nullptr, nvh);
+  ScalarField udotx(static_cast<size_t>(nvh));
+  sphere_tangency(ctx, udotx.data(), sphere.vert_x, sphere.velocity_verts, nvh);
+  udotx.update_host();
+  Real tangency = 0;
+  for (Index i = 0; i < sphere.n_vertices_host(); ++i)
+    tangency = std::fmax(tangency, std::fabs(udotx.host()[i]));
   const int nsteps = static_cast<int>(std::lround(a.tf / a.dt));
   std::vector<Real> x0(3 * sphere.n_vertices_host());
   for (Index i = 0; i < sphere.n_vertices_host(); ++i)
@@ -125,10 +145,12 @@ int run(const Args& a) {
       "{\"seed\": \"%s\", \"depth\": %d, \"n_verts\": %d, \"n_faces\": %d, "
       "\"renorm\": %d, \"steps\": %d, \"t\": %.6f, \"courant\": %.4f, "
       "\"velocity_rel_l2\": %.6e, \"stream_fn_rel_l2\": %.6e, "
+      "\"velocity_err_norms\": [%.6e, %.6e, %.6e], \"max_abs_u_dot_x\": %.6e, "
       "\"position_err_linf\": %.6e}\n",
       Seed::id_string().c_str(), a.depth, sphere.n_vertices_host(),
       sphere.n_faces_host(), a.renorm ? 1 : 0, nsteps, sphere.t, courant, std::sqrt(ev / nvv),
-      std::sqrt(es / ns), pos_err);
+      std::sqrt(es / ns), vel_norms.l1, vel_norms.l2, vel_norms.linf, tangency,
+      pos_err);
   return 0;
 }
 

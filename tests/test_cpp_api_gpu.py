@@ -38,6 +38,12 @@ def test_bve_rotation_driver(seed, renorm):
     # cube, from the quadrature) times the arc 2 pi t, plus the O(dt^2)
     # off-sphere stage error when not renormalising.
     assert out["position_err_linf"] < 1e-2 * 2 * 3.14159 * out["t"]
+    # ErrNorms on the device agrees with the host sum; the velocity is
+    # tangent to the sphere (SphereTangentFunctor) to rounding
+    l1, l2, linf = out["velocity_err_norms"]
+    assert abs(l2 - out["velocity_rel_l2"]) <= 1e-12 * out["velocity_rel_l2"]
+    assert 0 < l1 and 0 < linf
+    assert out["max_abs_u_dot_x"] < 1e-12 * 2 * 3.14159
 
 
 def test_bve_rotation_mixed_matches_fp64():
