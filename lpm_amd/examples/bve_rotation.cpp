@@ -20,6 +20,7 @@
 
 #include "lpm/bve_sphere.hpp"
 #include "lpm/error.hpp"
+#include "lpm/galleries.hpp"
 #include "lpm/vtk.hpp"
 
 using namespace Lpm;
@@ -56,14 +57,15 @@ Args parse(int argc, char** argv) {
 
 template <typename Seed>
 int run(const Args& a) {
-  constexpr Real OMG = 2 * constants::PI;
+  const SolidBodyRotation rotation;
+  constexpr Real OMG = SolidBodyRotation::OMEGA;
   MeshSeed<Seed> seed;
   Index nv, ne, nf;
   seed.set_max_allocations(nv, ne, nf, a.depth);
   BVESphere<Seed> sphere(nv, ne, nf);
   sphere.tree_init(a.depth, seed);
   sphere.set_omega(0);
-  sphere.init_vorticity([](Real, Real, Real z) { return 2 * OMG * z; });
+  sphere.init_vorticity(rotation);
   sphere.init_velocity(a.precision);
   sphere.init_stream_fn();
 
@@ -89,11 +91,10 @@ int run(const Args& a) {
   // the largest |u . x| (SphereTangentFunctor)
   const int64_t nvh = sphere.n_vertices_host();
   VectorField uex(static_cast<size_t>(nvh));
-  for (Index i = 0; i < sphere.n_vertices_host(); ++i) {
-    uex.host(i, 0) = -OMG * sphere.vert_x.host(i, 1);
-    uex.host(i, 1) = OMG * sphere.vert_x.host(i, 0);
-    uex.host(i, 2) = 0;
-  }
+  for (Index i = 0; i < sphere.n_vertices_host(); ++i)
+    rotation.init_velocity(uex.host(i, 0), uex.host(i, 1), uex.host(i, 2),
+                           sphere.vert_x.host(i, 0), sphere.vert_x.host(i, 1),
+                           sphere.vert_x.host(i, 2));
   uex.update_device();
   const Context& ctx = *Context::default_context();
   const ErrNorms vel_norms(ctx, nullptr, sphere.velocity_verts, uex, nullptr,
