@@ -620,7 +620,9 @@ def cpu_baseline(mesh, z_faces, budget_s=12.0):
         "cores": threads,
         "kind": "port",
         "sample": (f"one FP64 velocity evaluation at {len(sample)} of "
-                   f"{mesh.n_verts} vertex targets (every {stride}th) over all "
+                   f"{mesh.n_verts} vertex targets ("
+                   + ("the first" if stride == 1 else f"every {stride}th")
+                   + ") over all "
                    f"{mesh.n_faces} stored faces ({mesh.n_leaves} undivided), "
                    f"reference-faithful loop, {dt:.1f} s; rate counts useful "
                    f"pairs ({useful:.3e}); loop iterations {iters:.3e}"),

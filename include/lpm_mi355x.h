@@ -43,6 +43,12 @@ typedef enum lpm_status {
   LPM_ERR_INTERNAL = -6
 } lpm_status;
 
+/* LPM_MIXED evaluates 1 - x.y as |x - y|^2 / 2, which holds for positions
+ * on the unit sphere: off it by delta = |x|^2 - 1 the kernel is off by
+ * about delta / 2.  The time steppers keep positions on the sphere only
+ * with LPM_RK4_RENORMALIZE; without it, stage positions leave the sphere by
+ * O((dt |u|)^2) and the mixed sums then differ from FP64 by that much
+ * (1e-5 relative at dt |u| = 0.007), not by the ~1e-7 of the pair kernel. */
 typedef enum lpm_precision {
   LPM_FP64 = 0,  /* FP64 pair arithmetic, the reference's formula order */
   LPM_MIXED = 1  /* FP32 pair kernel (|x-y|^2/2 form), FP64 carries/state */

@@ -40,7 +40,8 @@ HIP_FLAGS = [
 HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
 HIP_SOURCES = ["bve_sums.hip", "ensemble.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
                "swe_rk4.hip", "ic2d.hip", "diagnostics.hip"]
-EXAMPLES = ["bve_rotation", "sphere_swe_tc2"]
+EXAMPLES = ["bve_rotation", "sphere_swe_tc2", "sphere_rh54",
+            "sphere_gaussian_vortex"]
 
 
 def _sources():

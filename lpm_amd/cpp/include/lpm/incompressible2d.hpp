@@ -86,6 +86,29 @@ class Incompressible2D : public PolyMesh2d<SeedType> {
 
   lpm_ctx* ctx() const { return ctx_->get(); }
 
+  // Leaf-face totals of Incompressible2D (src/lpm_incompressible2d_impl.hpp:
+  // 91-137): sum zeta A, (1/2) sum zeta^2 A and (1/2) sum |u|^2 A.  They read
+  // the current device fields (copied to their host mirrors).
+  Real total_vorticity() {
+    rel_vort_active.update_host();
+    return leaf_sum([&](Index i) { return rel_vort_active.host()[i]; });
+  }
+  Real total_enstrophy() {
+    rel_vort_active.update_host();
+    return 0.5 * leaf_sum([&](Index i) {
+      const Real z = rel_vort_active.host()[i];
+      return z * z;
+    });
+  }
+  Real total_kinetic_energy() {
+    velocity_active.update_host();
+    return 0.5 * leaf_sum([&](Index i) {
+      const Real u0 = velocity_active.host(i, 0), u1 = velocity_active.host(i, 1),
+                 u2 = velocity_active.host(i, 2);
+      return u0 * u0 + u1 * u1 + u2 * u2;
+    });
+  }
+
   ScalarField rel_vort_passive, stream_fn_passive;
   VectorField velocity_passive;
   ScalarField rel_vort_active, stream_fn_active;
@@ -106,6 +129,15 @@ class Incompressible2D : public PolyMesh2d<SeedType> {
   }
 
   std::shared_ptr<Context> ctx_;
+
+ private:
+  template <typename Fn>
+  Real leaf_sum(const Fn& f) const {
+    Real s = 0;
+    for (Index i = 0; i < this->n_faces_host(); ++i)
+      if (!this->face_mask.host()[i]) s += f(i) * this->face_area.host()[i];
+    return s;
+  }
 };
 
 // Incompressible2DRK2 (src/lpm_incompressible2d_rk2_impl.hpp:75-172).

@@ -76,6 +76,24 @@ class Incompressible2D:
         solver.advance_timestep_impl(self)
         self.t = solver.t_idx * solver.dt
 
+    # Leaf-face totals (src/lpm_incompressible2d_impl.hpp:91-137), reduced
+    # on the device in FP64.
+    def _leaf_sum(self, v):
+        leaf = (self.face_mask == 0).to(torch.float64)
+        return float((v * self.face_area * leaf).sum())
+
+    def total_vorticity(self) -> float:
+        """sum zeta A over the leaf faces."""
+        return self._leaf_sum(self.rel_vort_active)
+
+    def total_enstrophy(self) -> float:
+        """(1/2) sum zeta^2 A over the leaf faces."""
+        return 0.5 * self._leaf_sum(self.rel_vort_active ** 2)
+
+    def total_kinetic_energy(self) -> float:
+        """(1/2) sum |u|^2 A over the leaf faces."""
+        return 0.5 * self._leaf_sum((self.velocity_active ** 2).sum(dim=0))
+
 
 class Incompressible2DRK2:
     """Heun's method with the direct sums at the predictor and the new

@@ -90,3 +90,50 @@ def test_bve_rotation_vtk_frames(tmp_path):
                    x[:, 2]], 1)
     err = np.linalg.norm(f2["points"] - xr, axis=1).max()
     assert err == pytest.approx(out["position_err_linf"], rel=1e-5)
+
+
+@pytest.mark.parametrize("seed", ["cube", "icos"])
+def test_sphere_rh54_driver(seed):
+    """Stationary RH 5-4 wave (u0 = Omega / 14) with Incompressible2D: the
+    velocity matches RossbyWave54Velocity to the discretisation error, and
+    the relative vorticity carried by the particles stays the exact
+    RossbyHaurwitz54 at their new positions."""
+    out = run("sphere_rh54", "-d", 4, "-n", 5, "-tf", 0.025, "-seed", seed)
+    assert out["steps"] == 5 and abs(out["t"] - 0.025) < 1e-15
+    assert abs(out["u0"] - 2 * 3.141592653589793 / 14) < 1e-15
+    i, f = out["initial"], out["final"]
+    assert i["vorticity_err"] == [0.0, 0.0, 0.0]
+    # level-4 discretisation error of the velocity (1.7e-2 cube, 9e-3 icos)
+    # and its effect on the carried vorticity over five steps (6e-4, 7e-4)
+    for r in (i, f):
+        assert 0 < r["velocity_err"][1] < 2.5e-2
+        assert 0 < r["velocity_err"][2] < 5e-2
+    assert 0 < f["vorticity_err"][1] < 2e-3
+    assert 0 < f["vorticity_err"][2] < 5e-3
+    # totals: the wave integrates to zero vorticity; enstrophy and energy
+    # of a stationary wave do not change beyond the time-stepping error
+    assert abs(i["total_vorticity"]) < 1e-12 * i["total_enstrophy"]
+    assert abs(f["total_enstrophy"] / i["total_enstrophy"] - 1) < 1e-6
+    assert abs(f["total_kinetic_energy"] / i["total_kinetic_energy"] - 1) < 1e-3
+    mixed = run("sphere_rh54", "-d", 4, "-n", 5, "-tf", 0.025, "-seed", seed,
+                "-precision", "mixed")
+    # the mixed sums (FP32 pairs and logarithm, ~2e-7 relative to FP64 at
+    # level 4, positions kept on the sphere) move the norms by about that
+    assert abs(mixed["final"]["velocity_err"][1] - f["velocity_err"][1]) < 1e-6
+    assert abs(mixed["final"]["vorticity_err"][1] - f["vorticity_err"][1]) < 1e-6
+
+
+def test_sphere_gaussian_vortex_driver():
+    """Gaussian vortex with Incompressible2D: zero mean vorticity after
+    set_gauss_const; absolute vorticity is carried exactly, so the relative
+    totals move only with the discrete integral of z; the velocity is
+    tangent to the sphere."""
+    out = run("sphere_gaussian_vortex", "-d", 4, "-n", 5, "-tf", 0.025)
+    vort, ens, ke = (out["total_vorticity"], out["total_enstrophy"],
+                     out["total_kinetic_energy"])
+    assert len(vort) == 6 and out["steps"] == 5
+    assert abs(vort[0]) < 1e-12 * ens[0]
+    assert max(abs(v) for v in vort) < 1e-5 * ens[0]
+    assert max(abs(e / ens[0] - 1) for e in ens) < 1e-5
+    assert max(abs(k / ke[0] - 1) for k in ke) < 1e-3
+    assert out["max_abs_u_dot_x"] < 1e-13
