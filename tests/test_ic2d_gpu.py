@@ -170,3 +170,23 @@ def test_mixed_rk2_tracks_fp64(ctx):
                   a.velocity_passive.cpu().numpy()) < 1e-6
     # positions integrate the ~1e-7 velocity difference: dt |u| 5 steps
     assert rel_l2(b.vert_x.cpu().numpy(), a.vert_x.cpu().numpy()) < 1e-8
+
+
+def test_leaf_face_totals(ctx):
+    """total_vorticity / total_enstrophy / total_kinetic_energy
+    (src/lpm_incompressible2d_impl.hpp:91-137) over the leaf faces."""
+    s = _model(ctx, 1, 3, 0.0, 2 * np.pi)
+    s.init_direct_sums()
+    st = _state(s)
+    leaf = st["fmask"] == 0
+    a = st["farea"] * leaf
+    assert np.isclose(s.total_vorticity(), np.sum(st["fzeta"] * a),
+                      rtol=1e-13, atol=1e-13)
+    assert np.isclose(s.total_enstrophy(), 0.5 * np.sum(st["fzeta"] ** 2 * a),
+                      rtol=1e-13)
+    assert np.isclose(s.total_kinetic_energy(),
+                      0.5 * np.sum((st["fu"] ** 2).sum(axis=1) * a), rtol=1e-13)
+    # divided faces (zero area) do not count even with garbage values
+    s.rel_vort_active[torch.from_numpy(~leaf).to(s.rel_vort_active.device)] = 1e30
+    assert np.isclose(s.total_vorticity(), np.sum(st["fzeta"] * a),
+                      rtol=1e-13, atol=1e-13)
