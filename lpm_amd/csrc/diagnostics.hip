@@ -25,7 +25,7 @@ namespace lpm_amd {
 namespace {
 
 constexpr int kDiagBlock = 256;
-constexpr int kDiagMaxBlocks = 1024;
+constexpr int kDiagMaxBlocks = 4096;
 
 struct NormSums {
   double l1n, l1d, l2n, l2d, lin, lid;
