@@ -520,6 +520,41 @@ def ic2d_run(ctx, steps=10, warmup=1, dt=0.005, level=6, oracle_stride=64):
     return out
 
 
+def diagnostics_run(ctx, n=50_000_000, reps=10):
+    """ErrNorms and the tangency dot product (lpm_err_norms,
+    lpm_sphere_tangency) over n three-component FP64 fields with weights:
+    HBM-bound, 56 B per element read (80 B with the error field written).
+    Wall time per blocking call (err_norms returns its norms to the host)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    kw = dict(dtype=torch.float64, device="cuda", generator=g)
+    a = torch.randn(3, n, **kw)
+    x = torch.randn(3, n, **kw)
+    w = torch.rand(n, **kw)
+    err = torch.empty_like(a)
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    res = {"elements": n, "reps": reps}
+    for name, fn, nbytes in (
+            ("err_norms", lambda: ctx.err_norms(a, x, weight=w), 56 * n),
+            ("err_norms_with_error_field",
+             lambda: ctx.err_norms(a, x, weight=w, err=err), 80 * n),
+            ("sphere_tangency", lambda: ctx.sphere_tangency(a, x, out=out),
+             56 * n)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = {"ms_per_call": round(dt * 1e3, 4),
+                     "algorithmic_GB_per_s": round(nbytes / dt / 1e9, 1)}
+    del a, x, w, err, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def stream_fn_run(ctx, mesh, reps=3, oracle_stride=512):
     """BVE stream function (BVESphere::init_stream_fn): the FP64 sum of
     s log(1 - x.y) at every vertex and leaf face of the workload mesh, one
@@ -696,6 +731,8 @@ def main(argv=None):
                     help="skip the Incompressible2D run")
     ap.add_argument("--no-stream-fn", action="store_true",
                     help="skip the BVE stream-function evaluation")
+    ap.add_argument("--no-diagnostics", action="store_true",
+                    help="skip the error-norm / tangency bandwidth run")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -841,6 +878,9 @@ def main(argv=None):
     if not a.no_stream_fn:
         log("stream function")
         result["stream_fn"] = stream_fn_run(ctx, mesh)
+    if not a.no_diagnostics:
+        log("diagnostics (error norms, tangency)")
+        result["diagnostics"] = diagnostics_run(ctx)
     if not a.no_ensemble:
         log("ensemble run (64 members)")
         result["ensemble"] = ensemble_run(ctx, a.ensemble_steps, 1,
