@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .device import Context, crd_args
+from .device import Context, crd_args, warn_if_mixed_off_sphere
 from .mesh import SphereMesh
 
 
@@ -120,6 +120,7 @@ class BVERK4:
         self.nverts = sph.n_vertices_host()
         self.nfaces = sph.n_faces_host()
         self.precision = precision
+        warn_if_mixed_off_sphere(precision, renormalize)
         flags = 0
         if reference_compat:
             flags |= _abi.LPM_RK4_REFERENCE_COMPAT
@@ -248,6 +249,7 @@ class BVEEnsembleRK4:
         self.dt = float(dt)
         self.Omega = float(Omega)
         self.precision = precision
+        warn_if_mixed_off_sphere(precision, renormalize)
         flags = 0
         if reference_compat:
             flags |= _abi.LPM_RK4_REFERENCE_COMPAT

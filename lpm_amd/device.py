@@ -325,3 +325,14 @@ class Context:
         self.check(self.L.lpm_timing_query(self.h, kernel_id, ctypes.byref(ms),
                                            ctypes.byref(n), ctypes.byref(pairs)))
         return ms.value, n.value, pairs.value
+
+
+def warn_if_mixed_off_sphere(precision, renormalize):
+    """Mixed-precision sums evaluate 1 - x.y as |x - y|^2 / 2, exact only for
+    positions on the unit sphere; a stepper that does not project its stage
+    positions back drifts off it by O((dt |u|)^2), and the mixed sums then
+    differ from FP64 by about that much."""
+    if precision == "mixed" and not renormalize:
+        import warnings
+        warnings.warn("mixed precision assumes positions on the unit sphere; "
+                      "pass renormalize=True to keep them there", stacklevel=3)

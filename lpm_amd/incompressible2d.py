@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .device import Context, crd_args
+from .device import Context, crd_args, warn_if_mixed_off_sphere
 from .mesh import SphereMesh
 
 
@@ -105,6 +105,7 @@ class Incompressible2DRK2:
         self.dt = float(dt)
         self.t_idx = 0
         self.precision = precision
+        warn_if_mixed_off_sphere(precision, renormalize)
         flags = 0
         if leaf_only:
             flags |= _abi.LPM_FLAG_SKIP_MASKED_TARGETS

@@ -173,3 +173,16 @@ def test_null_context_is_rejected():
                             0, 0, 0.0, 0, 0, None, 1, 1, None)
     assert rc == _abi.LPM_ERR_ARG
     assert L.lpm_timing_enable(None, 1) == _abi.LPM_ERR_ARG
+
+
+def test_mixed_stepper_without_renormalize_warns():
+    """Mixed sums assume positions on the unit sphere: a mixed stepper that
+    does not renormalise warns; FP64 or renormalised mixed steppers do not."""
+    import warnings
+    from lpm_amd.incompressible2d import Incompressible2DRK2
+    with pytest.warns(UserWarning, match="unit sphere"):
+        Incompressible2DRK2(0.01, None, precision="mixed")
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        Incompressible2DRK2(0.01, None, precision="mixed", renormalize=True)
+        Incompressible2DRK2(0.01, None)
