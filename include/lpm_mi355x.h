@@ -21,6 +21,12 @@
  *    available from lpm_last_error(ctx).  Nothing throws across the ABI.
  *  - A context is single-threaded (not reentrant); distinct contexts may be
  *    used from distinct host threads on distinct streams.
+ *  - A context's workspaces (packed sources, mask scan, RK stage buffers,
+ *    work queues) are shared by all its calls.  When a call arrives on a
+ *    different stream than the context's previous call, the library orders
+ *    it after the previous stream's work (an event wait, no host sync).  A
+ *    call on a stream that is being captured by the caller is not ordered
+ *    this way: order such streams yourself.
  */
 #ifndef LPM_MI355X_H
 #define LPM_MI355X_H
@@ -64,6 +70,9 @@ typedef enum lpm_precision {
                                      the unit sphere (FP64) */
 #define LPM_RK4_USE_GRAPH 0x40u   /* capture the step in a hipGraph once and
                                      replay it on later identical calls */
+/* Captured step graphs a context keeps (keyed by fields, dt, Omega, flags
+ * and stream); beyond this the least recently replayed one is destroyed. */
+#define LPM_GRAPH_CACHE_MAX 16
 
 typedef struct lpm_ctx lpm_ctx;
 
@@ -82,6 +91,10 @@ const char* lpm_last_error(const lpm_ctx* ctx);
 
 /* Blocks until all work the context issued on `stream` has finished. */
 int lpm_stream_synchronize(lpm_ctx* ctx, void* stream);
+
+/* Number of captured step graphs the context holds (LPM_RK4_USE_GRAPH),
+ * at most LPM_GRAPH_CACHE_MAX.  No reference counterpart. */
+int lpm_graph_cache_size(const lpm_ctx* ctx, int* n_graphs);
 
 /* -------------------------------------------------------------- mesh ---- */
 /* Uniformly refined spherical quadtree mesh (host memory).  Replaces

@@ -27,6 +27,7 @@ LPM_FLAG_SKIP_MASKED_TARGETS = 0x2
 LPM_RK4_REFERENCE_COMPAT = 0x10
 LPM_RK4_RENORMALIZE = 0x20
 LPM_RK4_USE_GRAPH = 0x40
+LPM_GRAPH_CACHE_MAX = 16
 
 LPM_SEED_ICOS_TRI = 0
 LPM_SEED_CUBED_SPHERE = 1
@@ -124,6 +125,7 @@ _SIGS = {
     "lpm_ctx_destroy": (c_int, [c_vp]),
     "lpm_last_error": (ctypes.c_char_p, [c_vp]),
     "lpm_stream_synchronize": (c_int, [c_vp, c_vp]),
+    "lpm_graph_cache_size": (c_int, [c_vp, c_intp]),
     "lpm_mesh_create": (c_int, [c_int, c_int, ctypes.POINTER(c_vp)]),
     "lpm_mesh_destroy": (c_int, [c_vp]),
     "lpm_mesh_sizes": (c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_i64p]),

@@ -53,6 +53,22 @@ void set_error(lpm_ctx* ctx, const std::string& msg) {
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+void order_stream(lpm_ctx* ctx, hipStream_t s) {
+  if (ctx->has_last_stream && ctx->last_stream == s) return;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  LPM_HIP_CHECK(hipStreamIsCapturing(s, &st));
+  if (st != hipStreamCaptureStatusNone) return;
+  if (ctx->has_last_stream) {
+    if (!ctx->order_event)
+      LPM_HIP_CHECK(hipEventCreateWithFlags(&ctx->order_event,
+                                            hipEventDisableTiming));
+    LPM_HIP_CHECK(hipEventRecord(ctx->order_event, ctx->last_stream));
+    LPM_HIP_CHECK(hipStreamWaitEvent(s, ctx->order_event, 0));
+  }
+  ctx->last_stream = s;
+  ctx->has_last_stream = true;
+}
+
 std::atomic<uint64_t> g_workspace_epoch{0};
 
 DevBuf::~DevBuf() {
@@ -97,7 +113,8 @@ int pick_variant(lpm_ctx* ctx, int family, int64_t n_targets) {
 
 TimedLaunch::TimedLaunch(lpm_ctx* c, int k, hipStream_t s, double p)
     : ctx(c), kid(k), stream(s), pairs(p) {
-  if (!ctx->timing) return;
+  // kid < 0: an untimed launch
+  if (kid < 0 || !ctx->timing) return;
   LPM_HIP_CHECK(hipEventCreate(&e0));
   LPM_HIP_CHECK(hipEventCreate(&e1));
   LPM_HIP_CHECK(hipEventRecord(e0, stream));
@@ -149,6 +166,9 @@ int lpm_ctx_create(int device, lpm_ctx** out) {
 int lpm_ctx_destroy(lpm_ctx* ctx) {
   if (!ctx) return LPM_OK;
   (void)hipSetDevice(ctx->device);
+  // captured graphs and workspaces may still be in use by queued work
+  (void)hipDeviceSynchronize();
+  if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
   for (auto& r : ctx->timing_rec)
     for (auto& ev : r.pending) {
       (void)hipEventDestroy(ev.first);
@@ -162,6 +182,13 @@ int lpm_stream_synchronize(lpm_ctx* ctx, void* stream) {
   return guarded(ctx, [&] {
     LPM_ARG_CHECK(ctx, "null context");
     LPM_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  });
+}
+
+int lpm_graph_cache_size(const lpm_ctx* ctx, int* n) {
+  return guarded(nullptr, [&] {
+    LPM_ARG_CHECK(ctx && n, "null argument");
+    *n = static_cast<int>(ctx->graphs.size());
   });
 }
 
@@ -316,6 +343,7 @@ int lpm_bve_velocity(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                     "collocated: targets must be the sources (n_tgt == n_src)");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, src_mask, n_src, s);
     PackedSources ps = pack_sources(ctx, CrdIn{src_x, src_es, src_cs},
                                     src_zeta, src_area, map, precision, s);
@@ -356,6 +384,7 @@ int lpm_bve_stream_fn(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                     "collocated: targets must be the sources (n_tgt == n_src)");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, src_mask, n_src, s);
     // the stream function is evaluated in FP64 for both precisions
     PackedSources ps = pack_sources(ctx, CrdIn{src_x, src_es, src_cs},

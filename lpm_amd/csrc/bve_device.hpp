@@ -606,15 +606,19 @@ __global__ __launch_bounds__(kPackTile) void pack_mfma_kernel(
     ws = static_cast<float>(2.0 * y.w);
   }
   mf_y[j] = make_float4(y0, y1, y2, 1.0f);
-  mf_q[j] = y0 * y0 + y1 * y1 + y2 * y2;
   mf_s[j] = ws;
-  // transposed GEMM2 operand: local index jj = 16 ch + 4 g + r
-  const int jj = threadIdx.x, ch = jj >> 4, g = (jj >> 2) & 3, r = jj & 3;
-  float* tt = mf_t + static_cast<int64_t>(blockIdx.x) * 1024 + (ch * 4 + g) * 16 + r;
-  tt[0] = ws * y0;
-  tt[4] = ws * y1;
-  tt[8] = ws * y2;
-  tt[12] = ws;
+  // q and the GEMM2 operand are read by the f32 matrix-core kernel only
+  // (null: not packed)
+  if (mf_q) mf_q[j] = y0 * y0 + y1 * y1 + y2 * y2;
+  if (mf_t) {
+    // transposed GEMM2 operand: local index jj = 16 ch + 4 g + r
+    const int jj = threadIdx.x, ch = jj >> 4, g = (jj >> 2) & 3, r = jj & 3;
+    float* tt = mf_t + static_cast<int64_t>(blockIdx.x) * 1024 + (ch * 4 + g) * 16 + r;
+    tt[0] = ws * y0;
+    tt[4] = ws * y1;
+    tt[8] = ws * y2;
+    tt[12] = ws;
+  }
 }
 
 // Block `blk` (256 targets, one per lane) of segment sg against nsrc

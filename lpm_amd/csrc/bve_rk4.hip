@@ -17,6 +17,8 @@ struct lpm_ctx::GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t epoch = 0;
+  uint64_t last_use = 0;          // LRU stamp
+  hipStream_t stream = nullptr;   // stream of the last replay
   ~GraphEntry() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -373,14 +375,24 @@ void run_maybe_graph(lpm_ctx* ctx, bool use_graph, hipStream_t s,
   LPM_ARG_CHECK(s != nullptr, "LPM_RK4_USE_GRAPH needs a non-default stream");
   // the key's prepare step allocates; nothing may allocate while capturing
   const std::string key = key_fn();
-  LPM_HIP_CHECK(hipStreamSynchronize(s));
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end() &&
       it->second->epoch != g_workspace_epoch.load()) {
+    // stale (a workspace it records was reallocated): retire it once its
+    // last replay has finished
+    LPM_HIP_CHECK(hipStreamSynchronize(it->second->stream));
     ctx->graphs.erase(it);
     it = ctx->graphs.end();
   }
   if (it == ctx->graphs.end()) {
+    // bounded cache: retire the least recently replayed graph
+    while (ctx->graphs.size() >= static_cast<size_t>(LPM_GRAPH_CACHE_MAX)) {
+      auto old = ctx->graphs.begin();
+      for (auto g = ctx->graphs.begin(); g != ctx->graphs.end(); ++g)
+        if (g->second->last_use < old->second->last_use) old = g;
+      LPM_HIP_CHECK(hipStreamSynchronize(old->second->stream));
+      ctx->graphs.erase(old);
+    }
     auto entry = std::make_shared<lpm_ctx::GraphEntry>();
     const bool timing = ctx->timing;
     ctx->timing = false;
@@ -401,6 +413,9 @@ void run_maybe_graph(lpm_ctx* ctx, bool use_graph, hipStream_t s,
     entry->epoch = g_workspace_epoch.load();
     it = ctx->graphs.emplace(key, entry).first;
   }
+  // a replay is stream-ordered: the host does not wait for it
+  it->second->last_use = ++ctx->graph_clock;
+  it->second->stream = s;
   LPM_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
 }
 
@@ -451,6 +466,7 @@ int lpm_bve_init_velocity(lpm_ctx* ctx, const lpm_bve_fields* f,
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
     single_eval(ctx, *f, precision, flags, s)(map, vert_crd(*f), face_crd(*f),
                                               f->face_zeta);
@@ -470,6 +486,7 @@ int lpm_bve_autotune(lpm_ctx* ctx, const lpm_bve_fields* f, int precision,
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const int family =
         precision == LPM_MIXED ? KID_BVE_VEL_MIXED : KID_BVE_VEL_F64;
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
@@ -494,6 +511,7 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                   "dt and Omega must be finite");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     auto prepare = [&]() -> uint64_t {
       const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
       pack_sources(ctx, face_crd(*f), f->face_zeta, f->face_area, map,
@@ -518,6 +536,7 @@ int lpm_bve_ensemble_init_velocity(lpm_ctx* ctx, const lpm_bve_ensemble* e,
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const lpm_bve_fields& f = e->fields;
     const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
     ensemble_eval(ctx, d, f, precision, flags, s)(map, vert_crd(f), face_crd(f),
@@ -540,6 +559,7 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
                   "dt and Omega must be finite");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const lpm_bve_fields& f = e->fields;
     auto prepare = [&]() -> uint64_t {
       build_rank(ctx, f.face_mask, f.n_faces, s);
@@ -564,6 +584,7 @@ int lpm_bve_ensemble_autotune(lpm_ctx* ctx, const lpm_bve_ensemble* e,
     check_precision(precision);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const lpm_bve_fields& f = e->fields;
     const int family =
         precision == LPM_MIXED ? KID_BVE_ENS_MIXED : KID_BVE_ENS_F64;

@@ -426,39 +426,52 @@ double count_pairs(const TargetSeg* segs, int nseg, int64_t nsrc_upper) {
 }  // namespace
 
 // --------------------------------------------------------------- hosts ---
-RankMap build_rank(lpm_ctx* ctx, const uint8_t* mask, int64_t n,
-                   hipStream_t s) {
+RankMap build_rank_into(const RankWs& ws, const uint8_t* mask, int64_t n,
+                        hipStream_t s) {
   LPM_ARG_CHECK(n >= 0 && n < (int64_t(1) << 31), "source count out of range");
   RankMap map;
   map.n = n;
   if (!mask) {
-    int32_t* c = static_cast<int32_t*>(ctx->count_ident.ensure(sizeof(int32_t)));
-    set_count_kernel<<<1, 1, 0, s>>>(c, static_cast<int32_t>(n));
+    set_count_kernel<<<1, 1, 0, s>>>(ws.count_ident, static_cast<int32_t>(n));
     LPM_HIP_CHECK(hipGetLastError());
-    map.d_count = c;
+    map.d_count = ws.count_ident;
     return map;
   }
   const int nb = static_cast<int>((n + kScanChunk - 1) / kScanChunk);
-  int32_t* counts = static_cast<int32_t*>(
-      ctx->block_counts.ensure(sizeof(int32_t) * (nb + 1)));
-  const size_t nn = sizeof(int32_t) * std::max<int64_t>(n, 1);
-  int32_t* rank = static_cast<int32_t*>(ctx->rank.ensure(nn));
-  int32_t* inv = static_cast<int32_t*>(ctx->inv.ensure(nn));
   if (nb > 0) {
-    mask_count_kernel<<<nb, kScanBlock, 0, s>>>(mask, n, counts);
+    mask_count_kernel<<<nb, kScanBlock, 0, s>>>(mask, n, ws.counts);
     LPM_HIP_CHECK(hipGetLastError());
   }
-  scan_counts_kernel<<<1, 1024, 0, s>>>(counts, nb);
+  scan_counts_kernel<<<1, 1024, 0, s>>>(ws.counts, nb);
   LPM_HIP_CHECK(hipGetLastError());
   if (nb > 0) {
-    mask_rank_kernel<<<nb, kScanBlock, 0, s>>>(mask, n, counts, rank, inv);
+    mask_rank_kernel<<<nb, kScanBlock, 0, s>>>(mask, n, ws.counts, ws.rank,
+                                               ws.inv);
     LPM_HIP_CHECK(hipGetLastError());
   }
-  map.rank = rank;
-  map.inv = inv;
-  map.d_count = counts + nb;
-  map.block_offsets = counts;
+  map.rank = ws.rank;
+  map.inv = ws.inv;
+  map.d_count = ws.counts + nb;
+  map.block_offsets = ws.counts;
   return map;
+}
+
+RankMap build_rank(lpm_ctx* ctx, const uint8_t* mask, int64_t n,
+                   hipStream_t s) {
+  LPM_ARG_CHECK(n >= 0 && n < (int64_t(1) << 31), "source count out of range");
+  RankWs ws;
+  if (!mask) {
+    ws.count_ident =
+        static_cast<int32_t*>(ctx->count_ident.ensure(sizeof(int32_t)));
+  } else {
+    const int nb = static_cast<int>((n + kScanChunk - 1) / kScanChunk);
+    ws.counts = static_cast<int32_t*>(
+        ctx->block_counts.ensure(sizeof(int32_t) * (nb + 1)));
+    const size_t nn = sizeof(int32_t) * std::max<int64_t>(n, 1);
+    ws.rank = static_cast<int32_t*>(ctx->rank.ensure(nn));
+    ws.inv = static_cast<int32_t*>(ctx->inv.ensure(nn));
+  }
+  return build_rank_into(ws, mask, n, s);
 }
 
 TargetSeg face_targets(CrdIn x, const RankMap& map, const uint8_t* mask,
@@ -481,70 +494,118 @@ TargetSeg face_targets(CrdIn x, const RankMap& map, const uint8_t* mask,
   return t;
 }
 
-PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
-                           const double* area, const RankMap& map,
-                           int precision, hipStream_t s) {
+PackedSources pack_sources_into(const PackWs& ws, unsigned layouts, CrdIn sx,
+                                const double* zeta, const double* area,
+                                const RankMap& map, int precision,
+                                hipStream_t s) {
   PackedSources ps;
   const int64_t n = map.n;
   ps.count = map.d_count;
   ps.capacity = n;
   const int64_t cap = std::max<int64_t>(n, 1);
-  double4* p64 =
-      static_cast<double4*>(ctx->packed64.ensure(sizeof(double4) * cap));
-  double4* psy =
-      precision == LPM_FP64
-          ? static_cast<double4*>(ctx->packed64sy.ensure(sizeof(double4) * cap))
-          : nullptr;
+  const bool sy = precision == LPM_FP64 && (layouts & kLayoutSy);
+  LPM_ARG_CHECK(ws.p64 && (!sy || ws.psy), "packing buffers missing");
   if (n > 0) {
     pack_f64_kernel<<<static_cast<int>((n + 255) / 256), 256, 0, s>>>(
-        sx.base, sx.es, sx.cs, zeta, area, map.inv, map.d_count, p64, psy);
+        sx.base, sx.es, sx.cs, zeta, area, map.inv, map.d_count, ws.p64,
+        sy ? ws.psy : nullptr);
     LPM_HIP_CHECK(hipGetLastError());
   }
-  ps.data = p64;
-  ps.sy = psy;
-  if (precision == LPM_MIXED) {
-    const int ntile = static_cast<int>((cap + kPackTile - 1) / kPackTile);
-    float4* p32 =
-        static_cast<float4*>(ctx->packed32.ensure(sizeof(float4) * cap));
-    double4* cen = static_cast<double4*>(
-        ctx->centers.ensure(sizeof(double4) * ntile));
-    pack_mixed_kernel<<<ntile, kPackTile, 0, s>>>(p64, map.d_count, p32, cen);
+  ps.data = ws.p64;
+  ps.sy = sy ? ws.psy : nullptr;
+  if (precision != LPM_MIXED) return ps;
+  const int ntile = static_cast<int>((cap + kPackTile - 1) / kPackTile);
+  if (layouts & kLayoutVec32) {
+    LPM_ARG_CHECK(ws.p32 && ws.centers, "packing buffers missing");
+    pack_mixed_kernel<<<ntile, kPackTile, 0, s>>>(ws.p64, map.d_count, ws.p32,
+                                                  ws.centers);
     LPM_HIP_CHECK(hipGetLastError());
-    ps.data = p32;
-    ps.centers = cen;
-    const int64_t tcap = static_cast<int64_t>(ntile) * kPackTile;
-    MfmaSrc m;
-    m.y = static_cast<float4*>(ctx->mf_y.ensure(sizeof(float4) * tcap));
-    m.q = static_cast<float*>(ctx->mf_q.ensure(sizeof(float) * tcap));
-    m.s = static_cast<float*>(ctx->mf_s.ensure(sizeof(float) * tcap));
-    m.t = static_cast<float*>(ctx->mf_t.ensure(sizeof(float) * 4 * tcap));
-    m.c = static_cast<double4*>(ctx->mf_c.ensure(sizeof(double4) * ntile));
+    ps.data = ws.p32;
+    ps.centers = ws.centers;
+  }
+  if (layouts & (kLayoutMfma | kLayoutMfma16)) {
+    const bool full = (layouts & kLayoutMfma) != 0;
+    LPM_ARG_CHECK(ws.mf_y && ws.mf_s && ws.mf_c &&
+                      (!full || (ws.mf_q && ws.mf_t)),
+                  "packing buffers missing");
     pack_mfma_kernel<<<ntile, kPackTile, 0, s>>>(
-        p64, map.d_count, const_cast<float4*>(m.y), const_cast<float*>(m.q),
-        const_cast<float*>(m.s), const_cast<float*>(m.t),
-        const_cast<double4*>(m.c));
+        ws.p64, map.d_count, ws.mf_y, full ? ws.mf_q : nullptr, ws.mf_s,
+        full ? ws.mf_t : nullptr, ws.mf_c);
     LPM_HIP_CHECK(hipGetLastError());
-    ps.mf_y = m.y;
-    ps.mf_q = m.q;
-    ps.mf_s = m.s;
-    ps.mf_t = m.t;
-    ps.mf_c = m.c;
-    uint4* a16 = static_cast<uint4*>(ctx->mf_a.ensure(sizeof(uint4) * 2 * tcap));
-    float* t32 = static_cast<float*>(ctx->mf_t32.ensure(sizeof(float) * 4 * tcap));
-    double* is = static_cast<double*>(ctx->mf_is.ensure(sizeof(double) * ntile));
-    pack_mfma16_kernel<<<ntile, kPackTile, 0, s>>>(p64, map.d_count, m.c, a16,
-                                                   t32, is);
+    ps.mf_y = ws.mf_y;
+    ps.mf_s = ws.mf_s;
+    ps.mf_c = ws.mf_c;
+    if (full) {
+      ps.mf_q = ws.mf_q;
+      ps.mf_t = ws.mf_t;
+    }
+  }
+  if (layouts & kLayoutMfma16) {
+    LPM_ARG_CHECK(ws.mf_a && ws.mf_t32 && ws.mf_is, "packing buffers missing");
+    pack_mfma16_kernel<<<ntile, kPackTile, 0, s>>>(ws.p64, map.d_count,
+                                                   ws.mf_c, ws.mf_a, ws.mf_t32,
+                                                   ws.mf_is);
     LPM_HIP_CHECK(hipGetLastError());
-    ps.mf_a = a16;
-    ps.mf_t32 = t32;
-    ps.mf_is = is;
+    ps.mf_a = ws.mf_a;
+    ps.mf_t32 = ws.mf_t32;
+    ps.mf_is = ws.mf_is;
   }
   return ps;
 }
 
+// Context-owned buffers for every layout, so that any variant (tuned, forced
+// or default) finds its sources packed.
+PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
+                           const double* area, const RankMap& map,
+                           int precision, hipStream_t s) {
+  const int64_t cap = std::max<int64_t>(map.n, 1);
+  PackWs ws;
+  ws.p64 = static_cast<double4*>(ctx->packed64.ensure(sizeof(double4) * cap));
+  unsigned layouts = 0;
+  if (precision == LPM_FP64) {
+    ws.psy = static_cast<double4*>(ctx->packed64sy.ensure(sizeof(double4) * cap));
+    layouts = kLayoutSy;
+  } else if (precision == LPM_MIXED) {
+    const int ntile = static_cast<int>((cap + kPackTile - 1) / kPackTile);
+    const int64_t tcap = static_cast<int64_t>(ntile) * kPackTile;
+    ws.p32 = static_cast<float4*>(ctx->packed32.ensure(sizeof(float4) * cap));
+    ws.centers =
+        static_cast<double4*>(ctx->centers.ensure(sizeof(double4) * ntile));
+    ws.mf_y = static_cast<float4*>(ctx->mf_y.ensure(sizeof(float4) * tcap));
+    ws.mf_q = static_cast<float*>(ctx->mf_q.ensure(sizeof(float) * tcap));
+    ws.mf_s = static_cast<float*>(ctx->mf_s.ensure(sizeof(float) * tcap));
+    ws.mf_t = static_cast<float*>(ctx->mf_t.ensure(sizeof(float) * 4 * tcap));
+    ws.mf_c = static_cast<double4*>(ctx->mf_c.ensure(sizeof(double4) * ntile));
+    ws.mf_a = static_cast<uint4*>(ctx->mf_a.ensure(sizeof(uint4) * 2 * tcap));
+    ws.mf_t32 =
+        static_cast<float*>(ctx->mf_t32.ensure(sizeof(float) * 4 * tcap));
+    ws.mf_is = static_cast<double*>(ctx->mf_is.ensure(sizeof(double) * ntile));
+    layouts = kLayoutVec32 | kLayoutMfma | kLayoutMfma16;
+  }
+  return pack_sources_into(ws, layouts, sx, zeta, area, map, precision, s);
+}
+
+int bve_velocity_variant(lpm_ctx* ctx, int precision, int64_t n_targets) {
+  return pick_variant(
+      ctx, precision == LPM_MIXED ? KID_BVE_VEL_MIXED : KID_BVE_VEL_F64,
+      n_targets);
+}
+
+unsigned bve_variant_layouts(int precision, int variant) {
+  if (precision == LPM_FP64) {
+    LPM_ARG_CHECK(variant >= 0 && variant < kNumF64, "variant out of range");
+    return kF64Variants[variant].sy ? kLayoutSy : 0u;
+  }
+  LPM_ARG_CHECK(variant >= 0 && variant < kNumMixed, "variant out of range");
+  const MixedVariant& v = kMixedVariants[variant];
+  if (v.m16) return kLayoutMfma16;
+  if (v.mfn) return kLayoutMfma;
+  return kLayoutVec32;
+}
+
 void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
-                         hipStream_t s) {
+                         hipStream_t s, int variant, bool timed) {
   LPM_ARG_CHECK(nseg >= 1 && nseg <= 2, "one or two target segments");
   TargetSeg empty;
   const TargetSeg& t0 = segs[0];
@@ -554,25 +615,26 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                   "target count out of range");
   if (t0.n + t1.n == 0) return;
   const double pairs = count_pairs(segs, nseg, src.capacity);
+  if (variant < 0) variant = bve_velocity_variant(ctx, precision, t0.n + t1.n);
   if (precision == LPM_FP64) {
-    const F64Variant& v =
-        kF64Variants[pick_variant(ctx, KID_BVE_VEL_F64, t0.n + t1.n)];
+    LPM_ARG_CHECK(variant < kNumF64, "variant out of range");
+    const F64Variant& v = kF64Variants[variant];
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
     LPM_ARG_CHECK(!v.sy || src.sy, "packed s y missing for the FP64 variant");
-    TimedLaunch tl(ctx, KID_BVE_VEL_F64, s, pairs);
+    TimedLaunch tl(ctx, timed ? KID_BVE_VEL_F64 : -1, s, pairs);
     hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block), 0, s,
                        static_cast<const double4*>(src.data), src.sy, src.count,
                        a0, a1, eps2);
     LPM_HIP_CHECK(hipGetLastError());
   } else if (precision == LPM_MIXED) {
-    const MixedVariant& v =
-        kMixedVariants[pick_variant(ctx, KID_BVE_VEL_MIXED, t0.n + t1.n)];
+    LPM_ARG_CHECK(variant < kNumMixed, "variant out of range");
+    const MixedVariant& v = kMixedVariants[variant];
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
-    TimedLaunch tl(ctx, KID_BVE_VEL_MIXED, s, pairs);
+    TimedLaunch tl(ctx, timed ? KID_BVE_VEL_MIXED : -1, s, pairs);
     if (v.m16) {
       LPM_ARG_CHECK(src.mf_a, "matrix-core layout not packed");
       Mfma16Src m{src.mf_y, src.mf_s, src.mf_a, src.mf_t32, src.mf_c, src.mf_is};
@@ -586,6 +648,7 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          dim3(v.block), 0, s, m, src.count, a0, a1,
                          static_cast<float>(2.0 * eps2));
     } else {
+      LPM_ARG_CHECK(src.centers, "mixed vector layout not packed");
       hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block),
                          0, s, static_cast<const float4*>(src.data),
                          src.centers, src.count, a0, a1,
@@ -615,7 +678,13 @@ void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
                   int64_t cs, int ncomp, hipStream_t s) {
   if (n == 0) return;
-  int32_t* flag = static_cast<int32_t*>(ctx->flag.ensure(sizeof(int32_t)));
+  check_finite_into(static_cast<int32_t*>(ctx->flag.ensure(sizeof(int32_t))),
+                    base, n, es, cs, ncomp, s);
+}
+
+void check_finite_into(int32_t* flag, const double* base, int64_t n,
+                       int64_t es, int64_t cs, int ncomp, hipStream_t s) {
+  if (n == 0) return;
   zero_int_kernel<<<1, 1, 0, s>>>(flag);
   finite_kernel<<<static_cast<int>((n + 255) / 256), 256, 0, s>>>(
       base, n, es, cs, ncomp, flag);

@@ -168,6 +168,7 @@ int lpm_err_norms(lpm_ctx* ctx, const double* appx, int64_t appx_es,
                   "lpm_err_norms: err strides must be > 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const int64_t want = (n + kDiagBlock - 1) / kDiagBlock;
     const int nb = static_cast<int>(
         std::max<int64_t>(1, std::min<int64_t>(want, kDiagMaxBlocks)));
@@ -206,6 +207,7 @@ int lpm_sphere_tangency(lpm_ctx* ctx, const double* u, int64_t u_es,
     if (n == 0) return LPM_OK;
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     tangency_kernel<<<static_cast<int>((n + 255) / 256), 256, 0, s>>>(
         u, u_es, u_cs, x, x_es, x_cs, n, out);
     LPM_HIP_CHECK(hipGetLastError());

@@ -434,6 +434,7 @@ int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                     "collocated: targets must be the sources (n_tgt == n_src)");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, src_mask, n_src, s);
     PackedSources ps = pack_sources(ctx, CrdIn{src_x, src_es, src_cs}, src_zeta,
                                     src_area, map, precision, s);
@@ -465,6 +466,7 @@ int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
     LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
     ic2d_eval(ctx, *f, CrdIn{f->vert_x, f->vert_x_es, f->vert_x_cs},
               CrdIn{f->face_x, f->face_x_es, f->face_x_cs}, f->face_zeta, map,
@@ -485,6 +487,7 @@ int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
     LPM_ARG_CHECK(eps >= 0 && std::isfinite(eps), "eps must be >= 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     Ic2dStage V = carve(ctx->ic2d_v, f->n_verts);
     Ic2dStage F = carve(ctx->ic2d_f, f->n_faces);
     run_maybe_graph(

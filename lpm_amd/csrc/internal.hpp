@@ -186,6 +186,13 @@ struct lpm_ctx {
   // graph cache for RK4 steps
   struct GraphEntry;
   std::map<std::string, std::shared_ptr<GraphEntry>> graphs;
+  uint64_t graph_clock = 0;  // LRU stamps of the graph cache
+
+  // stream of the last call that used the context's workspaces, and the
+  // event that orders a call on another stream after it
+  hipStream_t last_stream = nullptr;
+  bool has_last_stream = false;
+  hipEvent_t order_event = nullptr;
 
   // autotuned kernel variants, keyed by (arch, kernel, size bucket)
   std::map<std::string, int> tuned;
@@ -195,6 +202,11 @@ struct lpm_ctx {
 namespace lpm_amd {
 
 void set_error(lpm_ctx* ctx, const std::string& msg);
+
+// Orders this call's work on stream s after everything the context issued
+// on the stream of its previous call (the workspaces are shared by all of
+// a context's calls).  No-op when the stream is unchanged or capturing.
+void order_stream(lpm_ctx* ctx, hipStream_t s);
 
 // Kernel variants of a family (KID_*) and the autotuning table.
 int variant_count(int family);
@@ -221,17 +233,74 @@ RankMap build_rank(lpm_ctx* ctx, const uint8_t* mask, int64_t n,
 PackedSources pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
                            const double* area, const RankMap& map,
                            int precision, hipStream_t s);
+
+// The same prologue into caller-provided device buffers (no allocation).
+// Sizes for n stored faces: counts ceil(n / kMaskScanChunk) + 1, rank and
+// inv n each (unused without a mask), count_ident 1.
+struct RankWs {
+  int32_t* counts = nullptr;
+  int32_t* rank = nullptr;
+  int32_t* inv = nullptr;
+  int32_t* count_ident = nullptr;
+};
+RankMap build_rank_into(const RankWs& ws, const uint8_t* mask, int64_t n,
+                        hipStream_t s);
+
+// Source layouts of the velocity variants (bit set).  Every layout also
+// needs the FP64 packing p64.  With tcap = 256 * ceil(n / 256) and
+// ntile = tcap / 256 the arrays hold: p64, psy n double4; p32 n float4;
+// centers, mf_c ntile double4; mf_y tcap float4; mf_q, mf_s tcap float;
+// mf_t, mf_t32 4 tcap float; mf_a 2 tcap uint4; mf_is ntile double.
+enum : unsigned {
+  kLayoutSy = 1u,      // FP64 {s y} (psy)
+  kLayoutVec32 = 2u,   // mixed vector-ALU kernels (p32, centers)
+  kLayoutMfma = 4u,    // f32 matrix-core kernel (mf_y, mf_q, mf_s, mf_t, mf_c)
+  kLayoutMfma16 = 8u,  // f16-split kernels (mf_y, mf_s, mf_c, mf_a, mf_t32,
+                       // mf_is)
+};
+struct PackWs {
+  double4* p64 = nullptr;
+  double4* psy = nullptr;
+  float4* p32 = nullptr;
+  double4* centers = nullptr;
+  float4* mf_y = nullptr;
+  float* mf_q = nullptr;
+  float* mf_s = nullptr;
+  float* mf_t = nullptr;
+  double4* mf_c = nullptr;
+  uint4* mf_a = nullptr;
+  float* mf_t32 = nullptr;
+  double* mf_is = nullptr;
+};
+PackedSources pack_sources_into(const PackWs& ws, unsigned layouts, CrdIn sx,
+                                const double* zeta, const double* area,
+                                const RankMap& map, int precision,
+                                hipStream_t s);
+// Variant of the BVE velocity kernel a launch over n_targets would use, and
+// the source layouts it reads.
+int bve_velocity_variant(lpm_ctx* ctx, int precision, int64_t n_targets);
+unsigned bve_variant_layouts(int precision, int variant);
 // Face targets of a collocated sum: every stored face (self skipped by
 // rank) or, with skip_masked, only the undivided ones (compacted slots).
 TargetSeg face_targets(CrdIn x, const RankMap& map, const uint8_t* mask,
                        bool skip_masked, CrdOut out, double* out_scalar);
+// variant < 0: bve_velocity_variant's choice.  timed = false skips the
+// timing events.
 void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
-                         hipStream_t s);
+                         hipStream_t s, int variant = -1, bool timed = true);
 void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s);
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,
                   int64_t cs, int ncomp, hipStream_t s);
+// check_finite with a caller-provided device flag (one int32).
+void check_finite_into(int32_t* flag, const double* base, int64_t n,
+                       int64_t es, int64_t cs, int ncomp, hipStream_t s);
+
+// ---- allocation-free velocity (noalloc.hip) ----
+// Looks up the device's static scratch array (once per device; the device
+// must be current).  Failures are recorded and reported by the entry point.
+void noalloc_setup(int device);
 
 // ---- RK4 (bve_rk4.hip) ----
 void validate_bve_fields(const lpm_bve_fields* f);

@@ -90,7 +90,17 @@ struct SweStageArgs {
   double dt, Omega, g, a;
   int is_face;
   int store_k;  // 0: leave k[s] (position increment) at zero
+  int renorm;   // project the work positions onto the unit sphere
 };
+
+// x / |x| in FP64 (LPM_RK4_RENORMALIZE).
+__device__ __forceinline__ void project_unit(double& w0, double& w1,
+                                             double& w2) {
+  const double r = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+  w0 *= r;
+  w1 *= r;
+  w2 *= r;
+}
 
 // Tendencies (sphere branch), stage increments and the next work state.
 __global__ __launch_bounds__(256) void swe_stage_kernel(SweStageArgs A) {
@@ -121,9 +131,13 @@ __global__ __launch_bounds__(256) void swe_stage_kernel(SweStageArgs A) {
   A.hk[i] = dh;
   if (A.xw) {
     const double* y = A.x0 + i * A.x0es;
-    A.xw[i] = y[0] + A.a * k0;
-    A.xw[i + n] = y[A.x0cs] + A.a * k1;
-    A.xw[i + 2 * n] = y[2 * A.x0cs] + A.a * k2;
+    double w0 = y[0] + A.a * k0;
+    double w1 = y[A.x0cs] + A.a * k1;
+    double w2 = y[2 * A.x0cs] + A.a * k2;
+    if (A.renorm) project_unit(w0, w1, w2);
+    A.xw[i] = w0;
+    A.xw[i + n] = w1;
+    A.xw[i + 2 * n] = w2;
     A.zw[i] = A.zeta0[i] + A.a * dz;
     A.sw[i] = A.sigma0[i] + A.a * ds;
     const double hw = A.ha0[i] + A.a * dh;
@@ -156,6 +170,7 @@ struct SweUpdateArgs {
   int64_t n;
   int is_face;
   int order;  // 4: SWERK4Update weights; 2: Heun, x += (x1/2 + x2/2)
+  int renorm; // project the new positions onto the unit sphere
 };
 
 // SWERK4Update (or the RK2 update) and the surface of the new state.
@@ -166,12 +181,15 @@ __global__ __launch_bounds__(256) void swe_update_kernel(SweUpdateArgs A) {
   constexpr double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
   const int64_t n = A.n;
   double* x = A.x + i * A.xes;
+  double v[3];
   if (A.order == 2) {
     // KokkosBlas::update(0.5, k1, 0.5, k2, 1, state) of the reference's RK2
     for (int c = 0; c < 3; ++c) {
       const int64_t o = i + c * n;
-      x[c * A.xcs] += 0.5 * A.k[0][o] + 0.5 * A.k[1][o];
+      v[c] = x[c * A.xcs] + (0.5 * A.k[0][o] + 0.5 * A.k[1][o]);
     }
+    if (A.renorm) project_unit(v[0], v[1], v[2]);
+    for (int c = 0; c < 3; ++c) x[c * A.xcs] = v[c];
     A.zeta[i] += 0.5 * A.zk[0][i] + 0.5 * A.zk[1][i];
     A.sigma[i] += 0.5 * A.sk[0][i] + 0.5 * A.sk[1][i];
     const double h = A.ha[i] + (0.5 * A.hk[0][i] + 0.5 * A.hk[1][i]);
@@ -187,9 +205,11 @@ __global__ __launch_bounds__(256) void swe_update_kernel(SweUpdateArgs A) {
   }
   for (int c = 0; c < 3; ++c) {
     const int64_t o = i + c * n;
-    x[c * A.xcs] += sixth * (A.k[0][o] + A.k[3][o]) +
-                    third * (A.k[1][o] + A.k[2][o]);
+    v[c] = x[c * A.xcs] + (sixth * (A.k[0][o] + A.k[3][o]) +
+                           third * (A.k[1][o] + A.k[2][o]));
   }
+  if (A.renorm) project_unit(v[0], v[1], v[2]);
+  for (int c = 0; c < 3; ++c) x[c * A.xcs] = v[c];
   A.zeta[i] += sixth * (A.zk[0][i] + A.zk[3][i]) + third * (A.zk[1][i] + A.zk[2][i]);
   A.sigma[i] += sixth * (A.sk[0][i] + A.sk[3][i]) + third * (A.sk[1][i] + A.sk[2][i]);
   const double h = A.ha[i] + (sixth * (A.hk[0][i] + A.hk[3][i]) +
@@ -329,6 +349,7 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
       a.is_face = face ? 1 : 0;
       // the reference never assigns the stage-4 position increment
       a.store_k = (compat && stage == 3) ? 0 : 1;
+      a.renorm = (flags & LPM_RK4_RENORMALIZE) ? 1 : 0;
       swe_stage_kernel<<<nblk(n), 256, 0, s>>>(a);
       LPM_HIP_CHECK(hipGetLastError());
     }
@@ -359,6 +380,7 @@ void issue_swe_step(lpm_ctx* ctx, const lpm_swe_fields& f, double dt,
     u.n = n;
     u.is_face = face ? 1 : 0;
     u.order = order;
+    u.renorm = (flags & LPM_RK4_RENORMALIZE) ? 1 : 0;
     swe_update_kernel<<<nblk(n), 256, 0, s>>>(u);
     LPM_HIP_CHECK(hipGetLastError());
   }
@@ -487,6 +509,7 @@ int lpm_swe_sphere_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                     "collocated: targets must be the sources (n_tgt == n_src)");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, src_mask, n_src, s);
     SwePacked ps = swe_pack_sources(ctx, CrdIn{src_x, src_es, src_cs},
                                     src_zeta, src_sigma, src_area, src_surf,
@@ -521,6 +544,7 @@ int lpm_swe_init_sums(lpm_ctx* ctx, const lpm_swe_fields* f, double eps,
     LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
     sums_eval(ctx, *f, swe_vert_crd(*f), swe_face_crd(*f), f->face_zeta,
               f->face_sigma, f->face_area, map, eps, pse_eps, precision, flags,
@@ -541,6 +565,7 @@ int swe_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt, double Omega,
     validate(f);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     SweStage V = carve(ctx->swe_v, f->n_verts);
     SweStage F = carve(ctx->swe_f, f->n_faces);
     const SweEvalFn ev =
@@ -569,6 +594,7 @@ int swe_ensemble_step(lpm_ctx* ctx, const lpm_swe_ensemble* e, double dt,
     const EnsembleDesc d = validate_swe_ensemble(e);
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const lpm_swe_fields& f = e->fields;
     SweStage V = carve(ctx->swe_v, f.n_verts);
     SweStage F = carve(ctx->swe_f, f.n_faces);
@@ -618,6 +644,7 @@ int lpm_swe_ensemble_init_sums(lpm_ctx* ctx, const lpm_swe_ensemble* e,
     LPM_ARG_CHECK(eps >= 0 && pse_eps >= 0, "eps, pse_eps must be >= 0");
     LPM_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = as_stream(stream);
+    order_stream(ctx, s);
     const lpm_swe_fields& f = e->fields;
     const RankMap map = build_rank(ctx, f.face_mask, f.n_faces, s);
     swe_ensemble_sums(ctx, d, f, swe_vert_crd(f), swe_face_crd(f),

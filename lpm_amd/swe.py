@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .device import Context, crd_args
+from .device import Context, crd_args, warn_if_mixed_off_sphere
 from .mesh import SphereMesh
 
 
@@ -127,13 +127,16 @@ class SWERK4:
 
     def __init__(self, dt: float, swe: SWE, precision="fp64",
                  reference_compat=True, leaf_only=False, use_graph=False,
-                 stream=None):
+                 stream=None, renormalize=False):
         self.dt = float(dt)
         self.t_idx = 0
         self.precision = precision
+        warn_if_mixed_off_sphere(precision, renormalize)
         flags = 0
         if reference_compat:
             flags |= _abi.LPM_RK4_REFERENCE_COMPAT
+        if renormalize:
+            flags |= _abi.LPM_RK4_RENORMALIZE
         if leaf_only:
             flags |= _abi.LPM_FLAG_SKIP_MASKED_TARGETS
         if use_graph:
@@ -159,10 +162,10 @@ class SWERK2(SWERK4):
     ``gmls_params`` have no counterpart."""
 
     def __init__(self, dt: float, swe: SWE, precision="fp64", leaf_only=False,
-                 use_graph=False, stream=None):
+                 use_graph=False, stream=None, renormalize=False):
         super().__init__(dt, swe, precision, reference_compat=False,
                          leaf_only=leaf_only, use_graph=use_graph,
-                         stream=stream)
+                         stream=stream, renormalize=renormalize)
 
     def advance_timestep_impl(self, swe: SWE):
         if self._fields is None:

@@ -564,8 +564,17 @@ void ora_swe_tendencies(const double* x, const double* u, const double* zeta,
  * fh derived).  On entry u, ddot, laps hold the sums at the current state.
  * compat != 0 keeps the reference's stage-4 position increment at zero (the
  * reference never assigns passive_x4/active_x4, :348-400); compat == 0 uses
- * dt*u after the stage-4 sums.  Pinned by: steady TC2 state (closed forms),
- * RK4 order. */
+ * dt*u after the stage-4 sums.  renorm != 0 projects every work position
+ * and the updated positions onto the unit sphere (not done by the
+ * reference).  Pinned by: steady TC2 state (closed forms), fourth-order
+ * self-convergence on a perturbed TC2 state. */
+static void project_rows(double* x, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const double r = sqrt(dot3(x + 3 * i, x + 3 * i));
+    for (int c = 0; c < 3; ++c) x[3 * i + c] /= r;
+  }
+}
+
 void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
                       double* vu, double* vdd, double* vlap, double* vs,
                       int64_t nv, double* fx, double* fzeta, double* fsigma,
@@ -573,7 +582,7 @@ void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
                       double* fu, double* fdd, double* flap, double* fs,
                       const uint8_t* fmask, int64_t nf, double dt,
                       double Omega, double g, double eps, double pse_eps,
-                      int compat, int nthreads) {
+                      int compat, int renorm, int nthreads) {
   const double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
   const int64_t nvv = nv > 0 ? nv : 1, nff = nf > 0 ? nf : 1;
   double *vk[4], *vzk[4], *vsk[4], *vhk[4], *fk[4], *fzk[4], *fsk[4], *fak[4];
@@ -634,6 +643,10 @@ void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
         fs[i] = fh[i];
       }
     }
+    if (renorm) {
+      project_rows(vxw, nv);
+      project_rows(fxw, nf);
+    }
     ora_swe_sums(vxw, vs, nv, fxw, fzw, fsw, faw, fmask, fs, nf, 0, eps,
                  pse_eps, vu, vdd, vlap, nthreads);
     ora_swe_sums(fxw, fs, nf, fxw, fzw, fsw, faw, fmask, fs, nf, 1, eps,
@@ -671,6 +684,10 @@ void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
       fs[i] = fh[i];
     }
   }
+  if (renorm) {
+    project_rows(vx, nv);
+    project_rows(fx, nf);
+  }
   ora_swe_sums(vx, vs, nv, fx, fzeta, fsigma, farea, fmask, fs, nf, 0, eps,
                pse_eps, vu, vdd, vlap, nthreads);
   ora_swe_sums(fx, fs, nf, fx, fzeta, fsigma, farea, fmask, fs, nf, 1, eps,
@@ -692,7 +709,8 @@ void ora_swe_rk4_step(double* vx, double* vzeta, double* vsigma, double* vh,
  * state += increments1 / 2 + increments2 / 2 (:190-205), the surface and the
  * sums at the new state (:207-232).  The surface Laplacian is the PSE sum
  * (the reference evaluates a GMLS operator there).  Arguments as
- * ora_swe_rk4_step.  Pinned by: steady TC2 state, second-order agreement with
+ * ora_swe_rk4_step (renorm: work and updated positions projected onto the
+ * unit sphere).  Pinned by: steady TC2 state, second-order agreement with
  * the RK4 step (error ratio under dt halving), and, for dt -> 0, the stage
  * layout's first-order term. */
 void ora_swe_rk2_step(double* vx, double* vzeta, double* vsigma, double* vh,
@@ -702,7 +720,7 @@ void ora_swe_rk2_step(double* vx, double* vzeta, double* vsigma, double* vh,
                       double* fu, double* fdd, double* flap, double* fs,
                       const uint8_t* fmask, int64_t nf, double dt,
                       double Omega, double g, double eps, double pse_eps,
-                      int nthreads) {
+                      int renorm, int nthreads) {
   const int64_t nvv = nv > 0 ? nv : 1, nff = nf > 0 ? nf : 1;
   double* vx1 = (double*)calloc(3 * nvv, sizeof(double));
   double* vx2 = (double*)calloc(3 * nvv, sizeof(double));
@@ -743,6 +761,10 @@ void ora_swe_rk2_step(double* vx, double* vzeta, double* vsigma, double* vh,
       fs[i] = fh[i];
     }
   }
+  if (renorm) {
+    project_rows(vxw, nv);
+    project_rows(fxw, nf);
+  }
   /* stage 2 */
   ora_swe_sums(vxw, vs, nv, fxw, fw, fw + nff, fw + 2 * nff, fmask, fs, nf, 0,
                eps, pse_eps, vu, vdd, vlap, nthreads);
@@ -773,6 +795,10 @@ void ora_swe_rk2_step(double* vx, double* vzeta, double* vsigma, double* vh,
       fh[i] = fmass[i] / farea[i];
       fs[i] = fh[i];
     }
+  }
+  if (renorm) {
+    project_rows(vx, nv);
+    project_rows(fx, nf);
   }
   ora_swe_sums(vx, vs, nv, fx, fzeta, fsigma, farea, fmask, fs, nf, 0, eps,
                pse_eps, vu, vdd, vlap, nthreads);

@@ -232,7 +232,7 @@ def swe_sums(tx, ts, sx, szeta, ssigma, sarea, ss, smask=None,
 
 
 def swe_rk4_step(s: dict, dt, Omega, g, eps, pse_eps, compat=True,
-                 nthreads=0):
+                 renorm=False, nthreads=0):
     """Advance the sphere SWE state ``s`` (dict of numpy arrays) one step."""
     lib().ora_swe_rk4_step(
         _d(s["vx"]), _d(s["vzeta"]), _d(s["vsigma"]), _d(s["vh"]),
@@ -242,11 +242,13 @@ def swe_rk4_step(s: dict, dt, Omega, g, eps, pse_eps, compat=True,
         _d(s["flap"]), _d(s["fs"]), _u8(s.get("fmask")), _i64(len(s["fx"])),
         ctypes.c_double(dt), ctypes.c_double(Omega), ctypes.c_double(g),
         ctypes.c_double(eps), ctypes.c_double(pse_eps),
-        ctypes.c_int(int(compat)), ctypes.c_int(nthreads))
+        ctypes.c_int(int(compat)), ctypes.c_int(int(renorm)),
+        ctypes.c_int(nthreads))
     return s
 
 
-def swe_rk2_step(s: dict, dt, Omega, g, eps, pse_eps, nthreads=0):
+def swe_rk2_step(s: dict, dt, Omega, g, eps, pse_eps, renorm=False,
+                 nthreads=0):
     """Advance the sphere SWE state ``s`` one RK2 (Heun) step."""
     lib().ora_swe_rk2_step(
         _d(s["vx"]), _d(s["vzeta"]), _d(s["vsigma"]), _d(s["vh"]),
@@ -256,7 +258,7 @@ def swe_rk2_step(s: dict, dt, Omega, g, eps, pse_eps, nthreads=0):
         _d(s["flap"]), _d(s["fs"]), _u8(s.get("fmask")), _i64(len(s["fx"])),
         ctypes.c_double(dt), ctypes.c_double(Omega), ctypes.c_double(g),
         ctypes.c_double(eps), ctypes.c_double(pse_eps),
-        ctypes.c_int(nthreads))
+        ctypes.c_int(int(renorm)), ctypes.c_int(nthreads))
     return s
 
 

@@ -322,6 +322,104 @@ def test_rk4_graph_replays_match_eager(ctx):
             assert torch.equal(a.velocity_verts, b.velocity_verts), k
 
 
+def test_rk4_graph_replays_are_asynchronous(ctx):
+    """Ten graph-replayed steps issued back to back with no host
+    synchronisation in between equal ten eager steps bit for bit."""
+    from lpm_amd.bve import BVERK4
+    a = _sphere(ctx, 1, 3)
+    b = _sphere(ctx, 1, 3)
+    a.init_velocity()
+    b.init_velocity()
+    ra = BVERK4(0.01, a, reference_compat=True)
+    stream = torch.cuda.Stream()
+    rb = BVERK4(0.01, b, reference_compat=True, use_graph=True)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        ra.advance_timestep(a)
+    with torch.cuda.stream(stream):
+        for _ in range(10):
+            rb.advance_timestep(b)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    assert torch.equal(a.vert_x, b.vert_x)
+    assert torch.equal(a.face_x, b.face_x)
+    assert torch.equal(a.rel_vort_faces, b.rel_vort_faces)
+    assert torch.equal(a.velocity_faces, b.velocity_faces)
+
+
+@pytest.mark.parametrize("compat", [True, False])
+def test_rk4_graph_replay_matches_oracle(ctx, ora, compat):
+    """Graph-replayed steps against the oracle directly (not only against
+    eager steps)."""
+    from lpm_amd.bve import BVERK4
+    s = _sphere(ctx, 0, 3)
+    s.init_velocity()
+    st = _oracle_state(s)
+    solver = BVERK4(0.01, s, reference_compat=compat, use_graph=True)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            solver.advance_timestep(s)
+    stream.synchronize()
+    for _ in range(3):
+        ora.bve_rk4_step(st, 0.01, OMEGA, compat=compat)
+    leaf = s.mesh.face_mask == 0
+    assert rel_l2(aos(s.vert_x.cpu().numpy()), st["vx"]) < TOL64
+    assert rel_l2(aos(s.face_x.cpu().numpy())[leaf], st["fx"][leaf]) < TOL64
+    assert rel_l2(s.rel_vort_faces.cpu().numpy()[leaf], st["fzeta"][leaf]) < TOL64
+    assert rel_l2(aos(s.velocity_verts.cpu().numpy()), st["vu"]) < 1e-11
+
+
+def test_graph_cache_stays_bounded(ctx):
+    """Adaptive time steps (a new dt every step) capture a new graph each
+    time; the context keeps at most LPM_GRAPH_CACHE_MAX of them and the
+    results still equal eager steps."""
+    from lpm_amd.bve import BVERK4
+    a = _sphere(ctx, 1, 2)
+    b = _sphere(ctx, 1, 2)
+    a.init_velocity()
+    b.init_velocity()
+    stream = torch.cuda.Stream()
+    L = _abi.lib()
+    n = ctypes.c_int()
+    with torch.cuda.stream(stream):
+        for k in range(_abi.LPM_GRAPH_CACHE_MAX + 6):
+            dt = 0.001 * (1 + k)
+            BVERK4(dt, a).advance_timestep(a)
+            BVERK4(dt, b, use_graph=True).advance_timestep(b)
+            ctx.check(L.lpm_graph_cache_size(ctx.h, ctypes.byref(n)))
+            assert n.value <= _abi.LPM_GRAPH_CACHE_MAX
+    stream.synchronize()
+    assert n.value == _abi.LPM_GRAPH_CACHE_MAX
+    assert torch.equal(a.vert_x, b.vert_x)
+    assert torch.equal(a.velocity_faces, b.velocity_faces)
+
+
+def test_context_switching_streams_is_ordered(ctx):
+    """One context driven from two streams in turn, with no host
+    synchronisation: the library orders each call after the previous
+    stream's work, so the results equal one-stream steps."""
+    from lpm_amd.bve import BVERK4
+    a = _sphere(ctx, 0, 4)
+    b = _sphere(ctx, 0, 4)
+    a.init_velocity()
+    b.init_velocity()
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    ra = BVERK4(0.01, a)
+    for _ in range(4):
+        ra.advance_timestep(a)
+    rb = BVERK4(0.01, b)
+    for k in range(4):
+        with torch.cuda.stream(s1 if k % 2 == 0 else s2):
+            rb.advance_timestep(b)
+    s1.synchronize()
+    s2.synchronize()
+    torch.cuda.synchronize()
+    assert torch.equal(a.vert_x, b.vert_x)
+    assert torch.equal(a.velocity_faces, b.velocity_faces)
+
+
 def test_rk4_solid_body_rotation(ctx):
     from lpm_amd.bve import BVERK4
     Om_r = 0.5

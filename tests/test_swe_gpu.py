@@ -179,6 +179,61 @@ def test_swe_rk2_matches_oracle(ctx, ora, seed):
         assert rel_l2(got[k][leaf], st[k][leaf]) < 1e-11, k
 
 
+@pytest.mark.parametrize("order", [4, 2])
+def test_swe_renormalized_steps_match_oracle(ctx, ora, order):
+    """LPM_RK4_RENORMALIZE projects work and updated positions onto the
+    sphere in both SWE steppers, as the oracle does."""
+    from lpm_amd.swe import SWERK2, SWERK4
+    s = _swe(ctx, 1, 2)
+    s.init_direct_sums()
+    torch.cuda.synchronize()
+    st = _oracle_state(s)
+    leaf = st["fmask"] == 0
+    solver = (SWERK4(0.02, s, reference_compat=False, renormalize=True)
+              if order == 4 else SWERK2(0.02, s, renormalize=True))
+    for _ in range(3):
+        s.advance_timestep(solver)
+        if order == 4:
+            ora.swe_rk4_step(st, 0.02, OMEGA, s.g, s.eps, s.pse_eps,
+                             compat=False, renorm=True)
+        else:
+            ora.swe_rk2_step(st, 0.02, OMEGA, s.g, s.eps, s.pse_eps,
+                             renorm=True)
+    torch.cuda.synchronize()
+    got = _oracle_state(s)
+    for k in ("vx", "vzeta", "vsigma", "vh", "vu", "vdd", "vlap"):
+        assert rel_l2(got[k], st[k]) < 1e-11, k
+    for k in ("fx", "fzeta", "fsigma", "farea", "fu", "fdd", "flap", "fh"):
+        assert rel_l2(got[k][leaf], st[k][leaf]) < 1e-11, k
+    r = np.linalg.norm(got["fx"][leaf], axis=1)
+    assert np.max(np.abs(r - 1.0)) < 1e-14
+    assert np.max(np.abs(np.linalg.norm(got["vx"], axis=1) - 1.0)) < 1e-14
+
+
+def test_swe_mixed_renormalized_trajectory_tracks_fp64(ctx):
+    """Mixed-precision SWE RK4 with renormalisation stays within 1e-6 of the
+    FP64 trajectory (velocity) over several steps; positions stay on the
+    sphere."""
+    from lpm_amd.swe import SWERK4
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        s = _swe(ctx, 0, 4)
+        s.init_direct_sums(precision=prec)
+        solver = SWERK4(0.01, s, precision=prec, reference_compat=False,
+                        renormalize=True, leaf_only=True)
+        for _ in range(5):
+            s.advance_timestep(solver)
+        torch.cuda.synchronize()
+        runs[prec] = _oracle_state(s)
+    a, b = runs["fp64"], runs["mixed"]
+    leaf = a["fmask"] == 0
+    assert rel_l2(b["vu"], a["vu"]) < 1e-6
+    assert rel_l2(b["fu"][leaf], a["fu"][leaf]) < 1e-6
+    assert rel_l2(b["vx"], a["vx"]) < 1e-8
+    assert rel_l2(b["fh"][leaf], a["fh"][leaf]) < 1e-8
+    assert np.max(np.abs(np.linalg.norm(b["fx"][leaf], axis=1) - 1)) < 1e-14
+
+
 def test_swe_argument_errors(ctx):
     L = _abi.lib()
     x = dev(np.eye(3))
