@@ -395,6 +395,11 @@ def test_graph_cache_stays_bounded(ctx):
     assert torch.equal(a.velocity_faces, b.velocity_faces)
 
 
+def _same_bits(p, q):
+    """Equal values, with NaN where the other has NaN."""
+    return bool(((p == q) | (p.isnan() & q.isnan())).all())
+
+
 def test_context_switching_streams_is_ordered(ctx):
     """One context driven from two streams in turn, with no host
     synchronisation: the library orders each call after the previous
@@ -417,7 +422,12 @@ def test_context_switching_streams_is_ordered(ctx):
     s2.synchronize()
     torch.cuda.synchronize()
     assert torch.equal(a.vert_x, b.vert_x)
-    assert torch.equal(a.velocity_faces, b.velocity_faces)
+    # a divided icosahedral face whose corners are equidistant shares its
+    # centre with its central child, so its collocated velocity is 0/0 = NaN,
+    # as in the reference's sum over every stored face: NaNs must sit at the
+    # same faces, everything else bit-equal
+    assert _same_bits(a.velocity_faces, b.velocity_faces)
+    assert _same_bits(a.face_x, b.face_x)
 
 
 def test_rk4_solid_body_rotation(ctx):
