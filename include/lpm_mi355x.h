@@ -166,6 +166,31 @@ int lpm_bve_velocity(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                      unsigned flags, double* out_u, int64_t out_es,
                      int64_t out_cs, void* stream);
 
+/* Allocation-free form of lpm_bve_velocity: same arguments and the same
+ * kernel variant, so the output is bit-equal to lpm_bve_velocity's.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP stream capture, tight serving loops).
+ * Every workspace (mask scan, packed sources, finite flag) is carved from
+ * one static device array of at most 56 MiB per device, held until the
+ * process ends; the library looks up each device's copy when a context is
+ * first created on it.  The call makes no hipMalloc or hipFree.
+ * n_src may not exceed lpm_bve_velocity_noalloc_max_sources() (524288
+ * stored faces): larger calls return LPM_ERR_ARG.  All calls on one device
+ * share the array and it keeps the last call's contents (each call rewrites
+ * everything it reads): calls that overlap on one GPU race, so issue them
+ * on one stream or order them.  LPM_FLAG_CHECK_FINITE synchronises the
+ * stream (not usable under capture). */
+int lpm_bve_velocity_noalloc(lpm_ctx* ctx, const double* tgt_x,
+                             int64_t tgt_es, int64_t tgt_cs, int64_t n_tgt,
+                             const double* src_x, int64_t src_es,
+                             int64_t src_cs, const double* src_zeta,
+                             const double* src_area, const uint8_t* src_mask,
+                             int64_t n_src, int collocated, double eps,
+                             int precision, unsigned flags, double* out_u,
+                             int64_t out_es, int64_t out_cs, void* stream);
+/* Largest n_src lpm_bve_velocity_noalloc accepts. */
+int64_t lpm_bve_velocity_noalloc_max_sources(void);
+
 /* Stream function psi(x_i) = sum_j log(1 - x_i.y_j) (-zeta_j A_j) / (4 pi).
  * Replaces BVEVertexStreamFn / BVEFaceStreamFn with StreamReduceDistinct /
  * StreamReduceCollocated and greens_fn (src/lpm_bve_sphere_kernels.hpp:20-48,

@@ -39,7 +39,7 @@ HIP_FLAGS = [
 # host-only sources: strict IEEE operation order for the mesh generator
 HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
 HIP_SOURCES = ["bve_sums.hip", "ensemble.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
-               "swe_rk4.hip", "ic2d.hip", "diagnostics.hip"]
+               "swe_rk4.hip", "ic2d.hip", "diagnostics.hip", "noalloc.hip"]
 EXAMPLES = ["bve_rotation", "sphere_swe_tc2", "sphere_rh54",
             "sphere_gaussian_vortex"]
 

@@ -159,6 +159,8 @@ int lpm_ctx_create(int device, lpm_ctx** out) {
     ctx->num_cus = prop.multiProcessorCount;
     ctx->arch = prop.gcnArchName;
     ctx->last_error = "no error";
+    // the device's copy of the allocation-free scratch array (once per device)
+    noalloc_setup(device);
     *out = ctx.release();
   });
 }
