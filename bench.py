@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("Gpair-interactions/s + RK4 steps/s, mixed vs FP64 at stated error, "
           "1 GPU")
 F_BVE = 22  # flops per pair (canonical count)
+F_SWE = 80  # flops per pair of the shallow-water sums (canonical count)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector, AMD public figure
 
 
@@ -210,6 +211,16 @@ def ensemble_run(ctx, steps, warmup, dt=0.01, autotune=True):
         "graph": True,
         "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
         "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
+        "kernel_ms_per_evaluation": {"mixed": round(m["kernel_ms"], 3),
+                                     "fp64": round(f["kernel_ms"], 3)},
+        "roofline_fp64_sums": {
+            "flops_per_pair": F_SWE,
+            "achieved": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12, 3),
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12
+                          / PEAK_FP64_TFLOPS, 4)},
+        "mixed_speedup_vs_fp64_sums": round(f["kernel_ms"] / m["kernel_ms"], 4),
+        "variants": tuned or None,
         "mixed_vs_fp64_t0_vertices_rel_l2_linf": errors(m["u0"], f["u0"]),
         "mixed_vs_fp64_after_steps_vertices_rel_l2_linf": errors(m["uT"], f["uT"]),
         "mixed_vs_fp64_after_steps_positions_rel_l2_linf": errors(m["xT"], f["xT"]),
@@ -292,8 +303,38 @@ def precision_study(ctx, steps_list=(1, 50), dt=0.005, level=8,
     return out
 
 
+def swe_variant_sweep(ctx, e, prec, flags, reps=1):
+    """Times every kernel variant of the SWE ensemble sums on the current
+    state (one launch each after a warm-up) and keeps the fastest."""
+    import torch
+    from lpm_amd import _abi
+    kid = (_abi.LPM_KERNEL_SWE_F64 if prec == "fp64"
+           else _abi.LPM_KERNEL_SWE_MIXED)
+    names = ctx.variants(kid)
+    ms = {}
+    for v, name in enumerate(names):
+        ctx.set_variant(kid, v)
+        e.init_direct_sums(prec, flags)
+        torch.cuda.synchronize()
+        ctx.timing_enable(True)
+        for _ in range(reps):
+            e.init_direct_sums(prec, flags)
+        torch.cuda.synchronize()
+        t, n, _ = ctx.timing_query(kid)
+        ctx.timing_enable(False)
+        ms[name] = t / max(n, 1)
+    best = min(ms, key=ms.get)
+    ctx.set_variant(kid, names.index(best))
+    d = names[ctx.L.lpm_variant_default(kid)]
+    log(f"swe variants {prec}: " + ", ".join(f"{k} {v:.1f} ms"
+                                            for k, v in ms.items()))
+    return {"variant": best, "ms": round(ms[best], 3),
+            "fixed_tile_baseline": {"variant": d, "ms": round(ms[d], 3)},
+            "all_ms": {k: round(v, 3) for k, v in ms.items()}}
+
+
 def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
-                     oracle_stride=64):
+                     oracle_stride=64, tune=True):
     """Shallow water: 16 steady zonal (TC2) states on the icosahedral mesh of
     level 6 with seeded 1e-4 height noise, one persistent launch per
     evaluation, graph-captured RK4; mixed timed, FP64 for the error."""
@@ -332,9 +373,12 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
     stream = torch.cuda.Stream()
     res = {}
     pairs = None
+    tuned = {}
     for prec in ("mixed", "fp64"):
         e = build()
         pairs = e.useful_pairs()
+        if tune:
+            tuned[prec] = swe_variant_sweep(ctx, e, prec, skip)
         e.init_direct_sums(prec, skip)
         torch.cuda.synchronize()
         t0 = {k: getattr(e, k).cpu().numpy().copy() for k in (
@@ -352,7 +396,15 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
         el = time.perf_counter() - tb
         tT = {k: getattr(e, k).cpu().numpy().copy() for k in (
             "velocity_passive", "surf_passive", "vert_x", "div_passive")}
-        res[prec] = {"elapsed": el, "t0": t0, "tT": tT}
+        # kernel time of one evaluation (for the roofline of the sums)
+        ctx.timing_enable(True)
+        e.init_direct_sums(prec, skip)
+        torch.cuda.synchronize()
+        kms, kn, _ = ctx.timing_query(_abi.LPM_KERNEL_SWE_F64 if prec == "fp64"
+                                      else _abi.LPM_KERNEL_SWE_MIXED)
+        ctx.timing_enable(False)
+        res[prec] = {"elapsed": el, "t0": t0, "tT": tT,
+                     "kernel_ms": kms / max(kn, 1)}
         log(f"swe ensemble {prec}: {el / steps * 1e3:.1f} ms/step, "
             f"{4 * pairs * steps / el / 1e9:.1f} Gpairs/s")
         del e, r
@@ -375,6 +427,16 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
         "graph": True,
         "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
         "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
+        "kernel_ms_per_evaluation": {"mixed": round(m["kernel_ms"], 3),
+                                     "fp64": round(f["kernel_ms"], 3)},
+        "roofline_fp64_sums": {
+            "flops_per_pair": F_SWE,
+            "achieved": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12, 3),
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12
+                          / PEAK_FP64_TFLOPS, 4)},
+        "mixed_speedup_vs_fp64_sums": round(f["kernel_ms"] / m["kernel_ms"], 4),
+        "variants": tuned or None,
         "mixed_vs_fp64_t0": {
             "velocity_rel_l2_linf": errors(m["t0"]["velocity_passive"],
                                            f["t0"]["velocity_passive"]),
@@ -701,6 +763,11 @@ def copy_calibration(mib, reps=5):
     return 0
 
 
+# legs that --only selects (each has a --no-<leg> switch)
+LEGS = ["mixed", "stream-fn", "diagnostics", "ensemble", "precision-study",
+        "swe", "ic2d", "cpu-baseline"]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -718,6 +785,9 @@ def main(argv=None):
     ap.add_argument("--variant-f64", default=None,
                     help="force this FP64 kernel variant (implies no autotune "
                          "of the FP64 run)")
+    ap.add_argument("--variant-mixed", default=None,
+                    help="force this mixed kernel variant (implies no autotune "
+                         "of the mixed run)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -733,7 +803,20 @@ def main(argv=None):
                     help="skip the BVE stream-function evaluation")
     ap.add_argument("--no-diagnostics", action="store_true",
                     help="skip the error-norm / tangency bandwidth run")
+    ap.add_argument("--swe-steps", type=int, default=10,
+                    help="RK4 steps of each shallow-water ensemble run")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated legs to run after the FP64 run "
+                         f"(of {','.join(LEGS)}); the others are skipped")
     a = ap.parse_args(argv)
+    if a.only is not None:
+        keep = {x.strip() for x in a.only.split(",") if x.strip()}
+        unknown = keep - set(LEGS)
+        if unknown:
+            ap.error(f"unknown legs {sorted(unknown)}; have {LEGS}")
+        for leg in LEGS:
+            if leg not in keep:
+                setattr(a, "no_" + leg.replace("-", "_"), True)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding "
@@ -766,6 +849,13 @@ def main(argv=None):
             log(f"unknown FP64 variant {a.variant_f64}; have {names64}")
             return 2
         ctx.set_variant(kid64, names64.index(a.variant_f64))
+    kidm = _abi.LPM_KERNEL_BVE_VEL_MIXED
+    if a.variant_mixed is not None:
+        namesm = ctx.variants(kidm)
+        if a.variant_mixed not in namesm:
+            log(f"unknown mixed variant {a.variant_mixed}; have {namesm}")
+            return 2
+        ctx.set_variant(kidm, namesm.index(a.variant_mixed))
     log("FP64 run")
     r64 = run_precision(ctx, mesh, "fp64", a.steps, a.warmup, a.dt,
                         dict(reference_compat=True),
@@ -849,7 +939,7 @@ def main(argv=None):
         log("mixed run")
         rmx = run_precision(ctx, mesh, "mixed", a.steps, a.warmup, a.dt,
                             dict(reference_compat=True, renormalize=True),
-                            autotune=not a.no_autotune)
+                            autotune=not a.no_autotune and a.variant_mixed is None)
         mstep = rmx["elapsed_s"] / a.steps
         mval = 4 * pairs_eval * a.steps / rmx["elapsed_s"] / 1e9
         mavg = rmx["kernel_ms_total"] / max(rmx["launches"], 1)
@@ -892,7 +982,8 @@ def main(argv=None):
         result["precision_study"] = precision_study(ctx)
     if not a.no_swe:
         log("shallow-water ensemble (16 x level 6)")
-        result["swe_ensemble"] = swe_ensemble_run(ctx)
+        result["swe_ensemble"] = swe_ensemble_run(ctx, steps=a.swe_steps,
+                                                  tune=not a.no_autotune)
     if not a.no_ic2d:
         log("Incompressible2D (level 6)")
         result["incompressible2d"] = ic2d_run(ctx)

@@ -379,6 +379,8 @@ int variant_count(int family) {
   if (family == KID_BVE_VEL_MIXED) return kNumMixed;
   if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
     return ens_variant_count(family);
+  if (family == KID_SWE_F64 || family == KID_SWE_MIXED)
+    return swe_variant_count(family);
   return 0;
 }
 
@@ -386,6 +388,8 @@ const char* variant_name(int family, int v) {
   if (v < 0 || v >= variant_count(family)) return nullptr;
   if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
     return ens_variant_name(family, v);
+  if (family == KID_SWE_F64 || family == KID_SWE_MIXED)
+    return swe_variant_name(family, v);
   return family == KID_BVE_VEL_F64 ? kF64Variants[v].name
                                    : kMixedVariants[v].name;
 }
@@ -393,6 +397,7 @@ const char* variant_name(int family, int v) {
 int default_variant(int family) {
   if (family == KID_BVE_ENS_F64 || family == KID_BVE_ENS_MIXED)
     return ens_default_variant(family);
+  if (family == KID_SWE_F64 || family == KID_SWE_MIXED) return 0;
   return family == KID_BVE_VEL_F64 ? kDefaultF64 : kDefaultMixed;
 }
 

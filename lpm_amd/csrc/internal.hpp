@@ -371,6 +371,9 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
                        unsigned flags, hipStream_t s);
 
 // ---- sphere SWE sums (swe_sums.hip) ----
+// Kernel variants of KID_SWE_F64 / KID_SWE_MIXED (the default is 0).
+int swe_variant_count(int family);
+const char* swe_variant_name(int family, int v);
 struct SwePacked {
   const void* data = nullptr;       // 64 B per source
   const double4* bounds = nullptr;  // per 256-source tile: centre, radius^2

@@ -6,21 +6,26 @@
 // 723-780, 877-930) plus the Gaussian PSE Laplacian of LaplacianReducer
 // (src/lpm_pse.hpp:157-203) on chord distances.
 //
-// Per pair (target x, source y) with d = y - x, delta = 1 - x.y,
-// D = delta + eps^2, a = zeta A / 4pi, b = sigma A / 4pi:
+// Per pair (target x, source y) with t = x.y, D = 1 - t + eps^2,
+// a = zeta A / 4pi, b = sigma A / 4pi and g = y - t x (= (I - x x^T) y):
 //   alpha = a / D, beta = b / D, a' = alpha / D, b' = beta / D.
 // The reference's 12-tuple sums factor into per-target accumulators
-//   A0 = sum alpha, Vd = sum alpha d, Yd = sum a' d d^T, Wd = sum a' d,
-//   Bs = sum beta, Vs = sum beta d, Sbd = sum beta delta,
-//   Ys = sum b' d d^T, Ws = sum b' delta d, Sd2 = sum b' delta^2,
+//   Vy = sum alpha y,  Gd = sum a' g g^T,  Vg = sum beta g,
+//   Bt = sum beta t,   Gs = sum b' g g^T,
 // and the finalisation applies the target-only factors once:
-//   u  = -x cross Vd - Vs - x Sbd
-//   Mz = E(x) A0 + E(Vd) + [x]x (Yd - eps^2 Wd x^T)       (E_ab = e_abc v_c)
-//   Ms = (Bs - Sbd) P(x) + x (Vs + Sbd x)^T
-//        - (Ys + Ws x^T + x Ws^T + Sd2 x x^T)
+//   u  = -x cross Vy - Vg
+//   Mz = E(Vy) + [x]x Gd - (x cross Vy) x^T        (E_ab = e_abc v_c)
+//   Ms = Bt (I - x x^T) + x Vg^T - Gs
 //   M  = Mz + Ms  (the reference's packed gradient, whose vorticity part
 //                  carries grad_kzeta's opposite sign convention)
 //   ddot = sum_ab M_ab M_ba.
+// These are identities of the pair sums for any x (no |x| = 1 needed):
+// with d = y - x and delta = 1 - t, d + delta x = g, x cross d = x cross g,
+// d - eps^2 x = g - D x and a' D = alpha, so the vorticity gradient
+// sum a' (x cross d)(d - eps^2 x)^T is [x]x Gd - (x cross Vy) x^T, and the
+// divergence terms sum b' (d + delta x)(d + delta x)^T, sum beta (d + delta
+// x) and sum beta (1 - delta) are Gs, Vg and Bt.  Per pair that is 17 FMAs
+// of accumulation and one reciprocal (shared by four pairs in FP64).
 // The PSE Laplacian sum_j (s_j - s_i) A_j (4/pi) exp(-|x-y|^2/e^2) / e^4 over
 // x.y > 0 is accumulated as L = sum A eta (s_j - s_i) in FP64 by both
 // kernels (a second difference: its first-order terms cancel by symmetry,
@@ -152,58 +157,102 @@ __device__ __forceinline__ int swe_self(const SweSeg& sg, int64_t c,
   return sg.self_identity ? static_cast<int>(c) : -1;
 }
 
-// Accumulators of one target (see the file comment).
-template <typename T>
+// FP64 accumulators of one target (see the file comment); Gd and Gs hold
+// the symmetric matrices as xx, xy, xz, yy, yz, zz.
 struct SweAcc {
-  T A0, Vd[3], Yd[6], Wd[3];  // Yd: xx, xy, xz, yy, yz, zz
-  T Bs, Vs[3], Sbd, Ys[6], Ws[3], Sd2;
-  T L1, L0;  // L1: PSE sum (FP64 accumulators only); L0 unused
+  double Vy[3], Gd[6], Vg[3], Bt, Gs[6];
+  double L1;  // PSE sum
   __device__ void zero() {
-    A0 = Bs = Sbd = Sd2 = L1 = L0 = T(0);
-    for (int k = 0; k < 3; ++k) Vd[k] = Wd[k] = Vs[k] = Ws[k] = T(0);
-    for (int k = 0; k < 6; ++k) Yd[k] = Ys[k] = T(0);
+    Bt = L1 = 0.0;
+    for (int k = 0; k < 3; ++k) Vy[k] = Vg[k] = 0.0;
+    for (int k = 0; k < 6; ++k) Gd[k] = Gs[k] = 0.0;
   }
 };
 
-// Velocity/gradient terms of one pair.  d: y - x; delta: 1 - x.y.
-template <typename T>
-__device__ __forceinline__ void swe_pair(SweAcc<T>& acc, const T d0,
-                                         const T d1, const T d2,
-                                         const T delta, const T r,
-                                         const T a, const T b) {
-  const T al = a * r, be = b * r;
-  const T ap = al * r, bp = be * r;
+// FP32 partials of one source tile: Vy is carried as A0 = sum alpha and
+// Vd = sum alpha d with d = y - x (Vy = x A0 + Vd), which keeps the
+// tile-relative precision of d.
+struct SweAcc32 {
+  float A0, Vd[3], Gd[6], Vg[3], Bt, Gs[6];
+  __device__ void zero() {
+    A0 = Bt = 0.0f;
+    for (int k = 0; k < 3; ++k) Vd[k] = Vg[k] = 0.0f;
+    for (int k = 0; k < 6; ++k) Gd[k] = Gs[k] = 0.0f;
+  }
+};
+
+// Gd += ap g g^T, Vg += be g, Bt += be t, Gs += bp g g^T.
+template <typename T, typename Acc>
+__device__ __forceinline__ void swe_pair_g(Acc& acc, const T g0, const T g1,
+                                           const T g2, const T t, const T be,
+                                           const T ap, const T bp) {
+  const T e0 = ap * g0, e1 = ap * g1, e2 = ap * g2;
+  acc.Gd[0] = fma(e0, g0, acc.Gd[0]);
+  acc.Gd[1] = fma(e0, g1, acc.Gd[1]);
+  acc.Gd[2] = fma(e0, g2, acc.Gd[2]);
+  acc.Gd[3] = fma(e1, g1, acc.Gd[3]);
+  acc.Gd[4] = fma(e1, g2, acc.Gd[4]);
+  acc.Gd[5] = fma(e2, g2, acc.Gd[5]);
+  acc.Vg[0] = fma(be, g0, acc.Vg[0]);
+  acc.Vg[1] = fma(be, g1, acc.Vg[1]);
+  acc.Vg[2] = fma(be, g2, acc.Vg[2]);
+  acc.Bt = fma(be, t, acc.Bt);
+  const T f0 = bp * g0, f1 = bp * g1, f2 = bp * g2;
+  acc.Gs[0] = fma(f0, g0, acc.Gs[0]);
+  acc.Gs[1] = fma(f0, g1, acc.Gs[1]);
+  acc.Gs[2] = fma(f0, g2, acc.Gs[2]);
+  acc.Gs[3] = fma(f1, g1, acc.Gs[3]);
+  acc.Gs[4] = fma(f1, g2, acc.Gs[4]);
+  acc.Gs[5] = fma(f2, g2, acc.Gs[5]);
+}
+
+// FP64 pair: source y, t = x.y, r = 1 / D.
+__device__ __forceinline__ void swe_pair_f64(SweAcc& acc, const double4& y,
+                                             const double t, const double r,
+                                             const double b, const double* x) {
+  const double al = y.w * r, be = b * r;
+  const double ap = al * r, bp = be * r;
+  acc.Vy[0] = fma(al, y.x, acc.Vy[0]);
+  acc.Vy[1] = fma(al, y.y, acc.Vy[1]);
+  acc.Vy[2] = fma(al, y.z, acc.Vy[2]);
+  const double g0 = fma(-t, x[0], y.x), g1 = fma(-t, x[1], y.y),
+               g2 = fma(-t, x[2], y.z);
+  swe_pair_g<double>(acc, g0, g1, g2, t, be, ap, bp);
+}
+
+// FP32 pair on tile-relative offsets: d = y - x, rho2 = |d|^2 (= 2 delta on
+// the unit sphere), r = 1 / D; xh = x / 2, so g = d + delta x = d + rho2 xh
+// and t = 1 - delta.
+__device__ __forceinline__ void swe_pair_f32(SweAcc32& acc, const float d0,
+                                             const float d1, const float d2,
+                                             const float rho2, const float r,
+                                             const float a, const float b,
+                                             const float* xh) {
+  const float al = a * r, be = b * r;
+  const float ap = al * r, bp = be * r;
   acc.A0 += al;
-  acc.Vd[0] = fma(al, d0, acc.Vd[0]);
-  acc.Vd[1] = fma(al, d1, acc.Vd[1]);
-  acc.Vd[2] = fma(al, d2, acc.Vd[2]);
-  const T e0 = ap * d0, e1 = ap * d1, e2 = ap * d2;
-  acc.Wd[0] += e0;
-  acc.Wd[1] += e1;
-  acc.Wd[2] += e2;
-  acc.Yd[0] = fma(e0, d0, acc.Yd[0]);
-  acc.Yd[1] = fma(e0, d1, acc.Yd[1]);
-  acc.Yd[2] = fma(e0, d2, acc.Yd[2]);
-  acc.Yd[3] = fma(e1, d1, acc.Yd[3]);
-  acc.Yd[4] = fma(e1, d2, acc.Yd[4]);
-  acc.Yd[5] = fma(e2, d2, acc.Yd[5]);
-  acc.Bs += be;
-  acc.Vs[0] = fma(be, d0, acc.Vs[0]);
-  acc.Vs[1] = fma(be, d1, acc.Vs[1]);
-  acc.Vs[2] = fma(be, d2, acc.Vs[2]);
-  acc.Sbd = fma(be, delta, acc.Sbd);
-  const T f0 = bp * d0, f1 = bp * d1, f2 = bp * d2;
-  acc.Ys[0] = fma(f0, d0, acc.Ys[0]);
-  acc.Ys[1] = fma(f0, d1, acc.Ys[1]);
-  acc.Ys[2] = fma(f0, d2, acc.Ys[2]);
-  acc.Ys[3] = fma(f1, d1, acc.Ys[3]);
-  acc.Ys[4] = fma(f1, d2, acc.Ys[4]);
-  acc.Ys[5] = fma(f2, d2, acc.Ys[5]);
-  const T bd = bp * delta;
-  acc.Ws[0] = fma(bd, d0, acc.Ws[0]);
-  acc.Ws[1] = fma(bd, d1, acc.Ws[1]);
-  acc.Ws[2] = fma(bd, d2, acc.Ws[2]);
-  acc.Sd2 = fma(bd, delta, acc.Sd2);
+  acc.Vd[0] = fmaf(al, d0, acc.Vd[0]);
+  acc.Vd[1] = fmaf(al, d1, acc.Vd[1]);
+  acc.Vd[2] = fmaf(al, d2, acc.Vd[2]);
+  const float g0 = fmaf(rho2, xh[0], d0), g1 = fmaf(rho2, xh[1], d1),
+              g2 = fmaf(rho2, xh[2], d2);
+  const float t = fmaf(-0.5f, rho2, 1.0f);
+  swe_pair_g<float>(acc, g0, g1, g2, t, be, ap, bp);
+}
+
+// Adds one tile's FP32 partials to the FP64 accumulators of target x.
+__device__ __forceinline__ void swe_carry(SweAcc& acc, const SweAcc32& p,
+                                          const double* x) {
+  for (int k = 0; k < 3; ++k) {
+    acc.Vy[k] += fma(x[k], static_cast<double>(p.A0),
+                     static_cast<double>(p.Vd[k]));
+    acc.Vg[k] += p.Vg[k];
+  }
+  acc.Bt += p.Bt;
+  for (int k = 0; k < 6; ++k) {
+    acc.Gd[k] += p.Gd[k];
+    acc.Gs[k] += p.Gs[k];
+  }
 }
 
 __device__ __forceinline__ void sym(const double* Y, double (*m)[3]) {
@@ -216,50 +265,40 @@ __device__ __forceinline__ void sym(const double* Y, double (*m)[3]) {
 }
 
 // u, ddot and the Laplacian from the accumulators of target x.
-__device__ __forceinline__ void swe_finalize(const SweAcc<double>& A, const double* x,
-                             double s_i, double eps2, double pse_scale,
-                             double* u, double* ddot, double* laps) {
-  u[0] = -(x[1] * A.Vd[2] - x[2] * A.Vd[1]) - A.Vs[0] - x[0] * A.Sbd;
-  u[1] = -(x[2] * A.Vd[0] - x[0] * A.Vd[2]) - A.Vs[1] - x[1] * A.Sbd;
-  u[2] = -(x[0] * A.Vd[1] - x[1] * A.Vd[0]) - A.Vs[2] - x[2] * A.Sbd;
+__device__ __forceinline__ void swe_finalize(const SweAcc& A, const double* x,
+                                             double pse_scale, double* u,
+                                             double* ddot, double* laps) {
+  const double* V = A.Vy;
+  const double xv[3] = {x[1] * V[2] - x[2] * V[1], x[2] * V[0] - x[0] * V[2],
+                        x[0] * V[1] - x[1] * V[0]};
+  u[0] = -xv[0] - A.Vg[0];
+  u[1] = -xv[1] - A.Vg[1];
+  u[2] = -xv[2] - A.Vg[2];
 
-  double Yd[3][3], Ys[3][3];
-  sym(A.Yd, Yd);
-  sym(A.Ys, Ys);
-  // G = Yd - eps^2 Wd x^T ; Mz = E(x) A0 + E(Vd) + [x]x G
-  double G[3][3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) G[a][b] = Yd[a][b] - eps2 * A.Wd[a] * x[b];
-  // [x]x G: row a = (x cross column)
+  double Gd[3][3], Gs[3][3];
+  sym(A.Gd, Gd);
+  sym(A.Gs, Gs);
+  // [x]x Gd: row a = (x cross column)_a
   double M[3][3];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    M[0][b] = x[1] * G[2][b] - x[2] * G[1][b];
-    M[1][b] = x[2] * G[0][b] - x[0] * G[2][b];
-    M[2][b] = x[0] * G[1][b] - x[1] * G[0][b];
+    M[0][b] = x[1] * Gd[2][b] - x[2] * Gd[1][b];
+    M[1][b] = x[2] * Gd[0][b] - x[0] * Gd[2][b];
+    M[2][b] = x[0] * Gd[1][b] - x[1] * Gd[0][b];
   }
-  const double v0 = x[0] * A.A0 + A.Vd[0], v1 = x[1] * A.A0 + A.Vd[1],
-               v2 = x[2] * A.A0 + A.Vd[2];
-  // E(v): [[0, v2, -v1], [-v2, 0, v0], [v1, -v0, 0]]
-  M[0][1] += v2;
-  M[0][2] -= v1;
-  M[1][0] -= v2;
-  M[1][2] += v0;
-  M[2][0] += v1;
-  M[2][1] -= v0;
-  const double s1 = A.Bs - A.Sbd;
-  const double w[3] = {A.Vs[0] + A.Sbd * x[0], A.Vs[1] + A.Sbd * x[1],
-                       A.Vs[2] + A.Sbd * x[2]};
+  // E(Vy): [[0, v2, -v1], [-v2, 0, v0], [v1, -v0, 0]]
+  M[0][1] += V[2];
+  M[0][2] -= V[1];
+  M[1][0] -= V[2];
+  M[1][2] += V[0];
+  M[2][0] += V[1];
+  M[2][1] -= V[0];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       const double P = (a == b ? 1.0 : 0.0) - x[a] * x[b];
-      M[a][b] += s1 * P + x[a] * w[b] -
-                 (Ys[a][b] + A.Ws[a] * x[b] + x[a] * A.Ws[b] +
-                  A.Sd2 * x[a] * x[b]);
+      M[a][b] += A.Bt * P + x[a] * A.Vg[b] - Gs[a][b] - xv[a] * x[b];
     }
   double dd = 0.0;
 #pragma unroll
@@ -270,38 +309,53 @@ __device__ __forceinline__ void swe_finalize(const SweAcc<double>& A, const doub
   *laps = pse_scale * A.L1;
 }
 
-// FP64 pair terms of one LDS tile (cnt sources, zero padding up to an even
-// count) for target x; `self` is the target's own source in tile numbering
-// (outside [0, cnt): none) and any_self says whether some lane of the wave
-// has one here.  Without, two sources share one reciprocal,
-// 1/D_a = D_b / (D_a D_b).
-__device__ __forceinline__ void swe_tile_f64(SweAcc<double>& acc,
-                                             const SweSrc* tile, int cnt,
-                                             const double* x, double eps2,
-                                             int self, bool any_self) {
+// FP64 pair terms of one LDS tile (cnt sources, zero padding up to a
+// multiple of SHARE) for target x; `self` is the target's own source in tile
+// numbering (outside [0, cnt): none) and any_self says whether some lane of
+// the wave has one here.  Without, SHARE sources share one reciprocal,
+// e.g. 1/D_a = D_b (D_c D_d) / (D_a D_b D_c D_d).
+__device__ __forceinline__ double swe_t(const double* x, const double4& y) {
+  return fma(x[2], y.z, fma(x[1], y.y, x[0] * y.x));
+}
+
+template <int SHARE = 4>
+__device__ __forceinline__ void swe_tile_f64(SweAcc& acc, const SweSrc* tile,
+                                             int cnt, const double* x,
+                                             double eps2, int self,
+                                             bool any_self) {
+  static_assert(SHARE == 2 || SHARE == 4, "two or four pairs per reciprocal");
   if (any_self) {
     for (int jj = 0; jj < cnt; ++jj) {
       const double4 yp = tile[jj].p;
-      const double4 yq = tile[jj].q;
-      const double d0 = yp.x - x[0], d1 = yp.y - x[1], d2 = yp.z - x[2];
-      const double t = x[0] * yp.x + x[1] * yp.y + x[2] * yp.z;
-      const double delta = 1.0 - t;
-      double r = rcp_nr<2>(delta + eps2);
+      const double t = swe_t(x, yp);
+      double r = rcp_nr<1>((1.0 - t) + eps2);
       if (jj == self) r = 0.0;
-      swe_pair<double>(acc, d0, d1, d2, delta, r, yp.w, yq.x);
+      swe_pair_f64(acc, yp, t, r, tile[jj].q.x, x);
+    }
+  } else if (SHARE == 2) {
+    for (int jj = 0; jj < cnt; jj += 2) {
+      const double4 ya = tile[jj].p, yb = tile[jj + 1].p;
+      const double ta = swe_t(x, ya), tb = swe_t(x, yb);
+      const double Da = (1.0 - ta) + eps2, Db = (1.0 - tb) + eps2;
+      const double r = rcp_nr<1>(Da * Db);
+      swe_pair_f64(acc, ya, ta, Db * r, tile[jj].q.x, x);
+      swe_pair_f64(acc, yb, tb, Da * r, tile[jj + 1].q.x, x);
     }
   } else {
-    for (int jj = 0; jj < cnt; jj += 2) {
-      const double4 ya = tile[jj].p, qa = tile[jj].q;
-      const double4 yb = tile[jj + 1].p, qb = tile[jj + 1].q;
-      const double a0 = ya.x - x[0], a1 = ya.y - x[1], a2 = ya.z - x[2];
-      const double b0 = yb.x - x[0], b1 = yb.y - x[1], b2 = yb.z - x[2];
-      const double da = 1.0 - (x[0] * ya.x + x[1] * ya.y + x[2] * ya.z);
-      const double db = 1.0 - (x[0] * yb.x + x[1] * yb.y + x[2] * yb.z);
-      const double Da = da + eps2, Db = db + eps2;
-      const double r = rcp_nr<2>(Da * Db);
-      swe_pair<double>(acc, a0, a1, a2, da, Db * r, ya.w, qa.x);
-      swe_pair<double>(acc, b0, b1, b2, db, Da * r, yb.w, qb.x);
+    for (int jj = 0; jj < cnt; jj += 4) {
+      const double4 ya = tile[jj].p, yb = tile[jj + 1].p;
+      const double4 yc = tile[jj + 2].p, yd = tile[jj + 3].p;
+      const double ta = swe_t(x, ya), tb = swe_t(x, yb);
+      const double tc = swe_t(x, yc), td = swe_t(x, yd);
+      const double Da = (1.0 - ta) + eps2, Db = (1.0 - tb) + eps2;
+      const double Dc = (1.0 - tc) + eps2, Dd = (1.0 - td) + eps2;
+      const double pab = Da * Db, pcd = Dc * Dd;
+      const double r = rcp_nr<1>(pab * pcd);
+      const double rab = pcd * r, rcd = pab * r;
+      swe_pair_f64(acc, ya, ta, Db * rab, tile[jj].q.x, x);
+      swe_pair_f64(acc, yb, tb, Da * rab, tile[jj + 1].q.x, x);
+      swe_pair_f64(acc, yc, tc, Dd * rcd, tile[jj + 2].q.x, x);
+      swe_pair_f64(acc, yd, td, Dc * rcd, tile[jj + 3].q.x, x);
     }
   }
 }
@@ -317,7 +371,7 @@ struct SweParams {
 
 // Block `blk` of segment `sg` against nsrc packed sources (bounds: their
 // tile bounding spheres).  Every thread of the block calls it.
-template <int BLOCK>
+template <int BLOCK, int SHARE>
 __device__ __forceinline__ void swe_f64_block(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     int nsrc, const SweSeg& sg, int blk, const SweParams& P, SweSrc* tile) {
@@ -334,7 +388,7 @@ __device__ __forceinline__ void swe_f64_block(
   const double s_i = (do_pse && sg.surf) ? sg.surf[i] : 0.0;
   const int self = swe_self(sg, c, i, in, P.skip_self);
 
-  SweAcc<double> acc;
+  SweAcc acc;
   acc.zero();
   for (int j0 = 0; j0 < nsrc; j0 += kSweTile) {
     __syncthreads();
@@ -351,7 +405,8 @@ __device__ __forceinline__ void swe_f64_block(
     const int cnt = min(kSweTile, nsrc - j0);
     const bool diag = static_cast<unsigned>(self - j0) <
                       static_cast<unsigned>(cnt);
-    swe_tile_f64(acc, tile, cnt, x, eps2, self - j0, __ballot(diag) != 0);
+    swe_tile_f64<SHARE>(acc, tile, cnt, x, eps2, self - j0,
+                        __ballot(diag) != 0);
     if (do_pse) {
       // wave-uniform test: is any target of the wave near this tile?
       const double4 bd = bounds[j0 / kSweTile];
@@ -374,8 +429,7 @@ __device__ __forceinline__ void swe_f64_block(
   }
   if (!in) return;
   double u[3], dd, lap;
-  swe_finalize(acc, x, s_i, eps2, k4OverPi * inv_pse_e2 * inv_pse_e2, u, &dd,
-               &lap);
+  swe_finalize(acc, x, k4OverPi * inv_pse_e2 * inv_pse_e2, u, &dd, &lap);
   if (sg.out_u) {
     double* o = sg.out_u + i * sg.oes;
     o[0] = u[0];
@@ -386,15 +440,17 @@ __device__ __forceinline__ void swe_f64_block(
   if (sg.out_laps && do_pse) sg.out_laps[i] = lap;
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_f64_kernel(
+// SHARE: pairs per FP64 reciprocal; MINW: minimum waves per SIMD asked of
+// the register allocator.
+template <int BLOCK, int SHARE, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void swe_f64_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
   __shared__ SweSrc tile[kSweTile];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SweSeg& sg = first ? s0 : s1;
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  swe_f64_block<BLOCK>(src, bounds, *count, sg, blk, P, tile);
+  swe_f64_block<BLOCK, SHARE>(src, bounds, *count, sg, blk, P, tile);
 }
 
 // ---------------------------------------------------------- mixed kernel --
@@ -427,7 +483,7 @@ __device__ __forceinline__ void swe_mixed_block(
   const int self = swe_self(sg, c, i, in, P.skip_self);
   const float e2f = static_cast<float>(eps2);
 
-  SweAcc<double> acc;
+  SweAcc acc;
   acc.zero();
   for (int j0 = 0; j0 < nsrc; j0 += kSweTile) {
     const double4 bd = bounds[j0 / kSweTile];
@@ -465,21 +521,28 @@ __device__ __forceinline__ void swe_mixed_block(
     // run in FP64 straight into the FP64 accumulators
     if (__ballot(in && g0 * g0 + g1 * g1 + g2 * g2 <=
                            kSweNearFp64 * kSweNearFp64 * bd.w)) {
-      swe_tile_f64(acc, tile64, cnt, x, eps2, self - j0, __ballot(diag) != 0);
+      swe_tile_f64<2>(acc, tile64, cnt, x, eps2, self - j0,
+                      __ballot(diag) != 0);
     } else {
-      SweAcc<float> p;
+      SweAcc32 p;
       p.zero();
-      // the per-pair self test only in tiles holding some lane's own source
+      const float xh[3] = {0.5f * static_cast<float>(x[0]),
+                           0.5f * static_cast<float>(x[1]),
+                           0.5f * static_cast<float>(x[2])};
+      // the per-pair self test only in tiles holding some lane's own source;
+      // the padding sources (far away, zero strengths) add nothing, so the
+      // loop runs to a multiple of four and unrolls
+      const int cnt4 = (cnt + 3) & ~3;
       auto pairs = [&](auto check) {
-        for (int jj = 0; jj < cnt; ++jj) {
+#pragma unroll 4
+        for (int jj = 0; jj < cnt4; ++jj) {
           const float4 yp = tile[jj].p;
           const float4 yq = tile[jj].q;
           const float d0 = yp.x - xr[0], d1 = yp.y - xr[1], d2 = yp.z - xr[2];
-          const float rho2 = d0 * d0 + d1 * d1 + d2 * d2;
-          const float delta = 0.5f * rho2;
-          float r = __builtin_amdgcn_rcpf(delta + e2f);
+          const float rho2 = fmaf(d2, d2, fmaf(d1, d1, d0 * d0));
+          float r = __builtin_amdgcn_rcpf(fmaf(0.5f, rho2, e2f));
           if (decltype(check)::value && j0 + jj == self) r = 0.0f;
-          swe_pair<float>(p, d0, d1, d2, delta, r, yp.w, yq.x);
+          swe_pair_f32(p, d0, d1, d2, rho2, r, yp.w, yq.x, xh);
         }
       };
       if (__ballot(diag))
@@ -487,21 +550,7 @@ __device__ __forceinline__ void swe_mixed_block(
       else
         pairs(std::false_type{});
       // carry the tile's FP32 partials into the FP64 accumulators
-      acc.A0 += p.A0;
-      acc.Bs += p.Bs;
-      acc.Sbd += p.Sbd;
-      acc.Sd2 += p.Sd2;
-
-      for (int k = 0; k < 3; ++k) {
-        acc.Vd[k] += p.Vd[k];
-        acc.Wd[k] += p.Wd[k];
-        acc.Vs[k] += p.Vs[k];
-        acc.Ws[k] += p.Ws[k];
-      }
-      for (int k = 0; k < 6; ++k) {
-        acc.Yd[k] += p.Yd[k];
-        acc.Ys[k] += p.Ys[k];
-      }
+      swe_carry(acc, p, x);
     }
     if (do_pse) {
       const double e0 = x[0] - bd.x, e1 = x[1] - bd.y, e2 = x[2] - bd.z;
@@ -527,8 +576,7 @@ __device__ __forceinline__ void swe_mixed_block(
   }
   if (!in) return;
   double u[3], dd, lap;
-  swe_finalize(acc, x, s_i, eps2, k4OverPi * inv_pse_e2 * inv_pse_e2, u, &dd,
-               &lap);
+  swe_finalize(acc, x, k4OverPi * inv_pse_e2 * inv_pse_e2, u, &dd, &lap);
   if (sg.out_u) {
     double* o = sg.out_u + i * sg.oes;
     o[0] = u[0];
@@ -539,11 +587,11 @@ __device__ __forceinline__ void swe_mixed_block(
   if (sg.out_laps && do_pse) sg.out_laps[i] = lap;
 }
 
-template <int BLOCK>
-// at least 3 waves per SIMD: the FP64 accumulators, the FP32 tile partials
-// and the PSE loop otherwise hold 198 registers (2 waves); the few spills
-// this costs sit outside the pair loop
-__global__ __launch_bounds__(BLOCK, 3) void swe_mixed_kernel(
+// MINW: minimum waves per SIMD asked of the register allocator (the FP64
+// accumulators, the FP32 tile partials and the FP64 near-tile and PSE loops
+// hold about 170 registers at 3 waves).
+template <int BLOCK, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void swe_mixed_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
   __shared__ F4x2 tile[kSweTile];
@@ -655,8 +703,8 @@ __device__ __forceinline__ int swe_ens_next(const SweEnsArgs& A, int* slot) {
   return it;
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void swe_ens_f64_kernel(
+template <int BLOCK, int SHARE, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void swe_ens_f64_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     SweEnsArgs A) {
   __shared__ SweSrc tile[kSweTile];
@@ -667,16 +715,13 @@ __global__ __launch_bounds__(BLOCK) void swe_ens_f64_kernel(
     const int4 w = A.items[it];
     const EnsMember M = A.mem[w.x];
     const SweSeg sg = swe_ens_segment(A, M, w.y);
-    swe_f64_block<BLOCK>(src + M.P, bounds + M.P / kSweTile, M.cnt, sg, w.z,
-                         A.P, tile);
+    swe_f64_block<BLOCK, SHARE>(src + M.P, bounds + M.P / kSweTile, M.cnt,
+                                sg, w.z, A.P, tile);
   }
 }
 
-template <int BLOCK>
-// at least 3 waves per SIMD: the FP64 accumulators, the FP32 tile partials
-// and the PSE loop otherwise hold 198 registers (2 waves); the few spills
-// this costs sit outside the pair loop
-__global__ __launch_bounds__(BLOCK, 3) void swe_ens_mixed_kernel(
+template <int BLOCK, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void swe_ens_mixed_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__ bounds,
     SweEnsArgs A) {
   __shared__ F4x2 tile[kSweTile];
@@ -691,6 +736,49 @@ __global__ __launch_bounds__(BLOCK, 3) void swe_ens_mixed_kernel(
     swe_mixed_block<BLOCK>(src + M.P, bounds + M.P / kSweTile, M.cnt, sg, w.z,
                            A.P, tile, tile64);
   }
+}
+
+// ------------------------------------------------------ kernel registry ---
+using SweKernel = void (*)(const SweSrc*, const double4*, const int32_t*,
+                           SweSeg, SweSeg, SweParams);
+using SweEnsKernel = void (*)(const SweSrc*, const double4*, SweEnsArgs);
+struct SweVariant {
+  SweKernel fn;
+  SweEnsKernel ens;
+  const char* name;
+};
+constexpr int kSweBlock = 256;
+const SweVariant kSweF64Variants[] = {
+    {swe_f64_kernel<kSweBlock, 4, 3>, swe_ens_f64_kernel<kSweBlock, 4, 3>,
+     "share4_occ3"},
+    {swe_f64_kernel<kSweBlock, 4, 4>, swe_ens_f64_kernel<kSweBlock, 4, 4>,
+     "share4_occ4"},
+    {swe_f64_kernel<kSweBlock, 2, 4>, swe_ens_f64_kernel<kSweBlock, 2, 4>,
+     "share2_occ4"},
+    {swe_f64_kernel<kSweBlock, 2, 3>, swe_ens_f64_kernel<kSweBlock, 2, 3>,
+     "share2_occ3"},
+};
+const SweVariant kSweMixedVariants[] = {
+    {swe_mixed_kernel<kSweBlock, 4>, swe_ens_mixed_kernel<kSweBlock, 4>,
+     "occ4"},
+    {swe_mixed_kernel<kSweBlock, 3>, swe_ens_mixed_kernel<kSweBlock, 3>,
+     "occ3"},
+    {swe_mixed_kernel<kSweBlock, 2>, swe_ens_mixed_kernel<kSweBlock, 2>,
+     "occ2"},
+};
+constexpr int kNumSweF64 = sizeof(kSweF64Variants) / sizeof(kSweF64Variants[0]);
+constexpr int kNumSweMixed =
+    sizeof(kSweMixedVariants) / sizeof(kSweMixedVariants[0]);
+
+const SweVariant& swe_variant(lpm_ctx* ctx, int precision, int64_t n_targets) {
+  const int family = precision == LPM_FP64 ? KID_SWE_F64 : KID_SWE_MIXED;
+  const int v = pick_variant(ctx, family, n_targets);
+  if (precision == LPM_FP64) {
+    LPM_ARG_CHECK(v >= 0 && v < kNumSweF64, "variant out of range");
+    return kSweF64Variants[v];
+  }
+  LPM_ARG_CHECK(v >= 0 && v < kNumSweMixed, "variant out of range");
+  return kSweMixedVariants[v];
 }
 
 SweSeg to_seg(const SweTargets& t, int per_block) {
@@ -714,6 +802,17 @@ SweSeg to_seg(const SweTargets& t, int per_block) {
 }
 
 }  // namespace
+
+int swe_variant_count(int family) {
+  return family == KID_SWE_F64 ? kNumSweF64
+                               : (family == KID_SWE_MIXED ? kNumSweMixed : 0);
+}
+
+const char* swe_variant_name(int family, int v) {
+  if (v < 0 || v >= swe_variant_count(family)) return nullptr;
+  return family == KID_SWE_F64 ? kSweF64Variants[v].name
+                               : kSweMixedVariants[v].name;
+}
 
 SwePacked swe_pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
                            const double* sigma, const double* area,
@@ -757,15 +856,12 @@ void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
   const int skip_self = eps == 0.0 ? 1 : 0;
   double pairs = static_cast<double>(t0.base.n + t1.base.n) * src.capacity;
   const SweParams P{eps * eps, inv_e2, cut2, do_pse, skip_self};
-  if (precision == LPM_FP64) {
-    TimedLaunch tl(ctx, KID_SWE_F64, s, pairs);
-    hipLaunchKernelGGL((swe_f64_kernel<kBlock>), dim3(grid), dim3(kBlock), 0, s,
+  const SweVariant& v = swe_variant(ctx, precision, t0.base.n + t1.base.n);
+  {
+    TimedLaunch tl(ctx, precision == LPM_FP64 ? KID_SWE_F64 : KID_SWE_MIXED, s,
+                   pairs);
+    hipLaunchKernelGGL(v.fn, dim3(grid), dim3(kBlock), 0, s,
                        static_cast<const SweSrc*>(src.data), src.bounds,
-                       src.count, a0, a1, P);
-  } else {
-    TimedLaunch tl(ctx, KID_SWE_MIXED, s, pairs);
-    hipLaunchKernelGGL((swe_mixed_kernel<kBlock>), dim3(grid), dim3(kBlock), 0,
-                       s, static_cast<const SweSrc*>(src.data), src.bounds,
                        src.count, a0, a1, P);
   }
   LPM_HIP_CHECK(hipGetLastError());
@@ -832,21 +928,14 @@ void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
                   do_pse ? kPseCut * pse_eps * pse_eps : 0.0, do_pse,
                   eps == 0.0 ? 1 : 0};
   ensemble_reset_queue(L, s);
-  if (precision == LPM_FP64) {
-    const int grid = ensemble_grid(
-        ctx, reinterpret_cast<const void*>(&swe_ens_f64_kernel<kBlock>),
-        kBlock, L.n_items);
-    TimedLaunch tl(ctx, KID_SWE_F64, s, L.pairs);
-    hipLaunchKernelGGL((swe_ens_f64_kernel<kBlock>), dim3(grid), dim3(kBlock),
-                       0, s, static_cast<const SweSrc*>(src),
-                       static_cast<const double4*>(bounds), A);
-  } else {
-    const int grid = ensemble_grid(
-        ctx, reinterpret_cast<const void*>(&swe_ens_mixed_kernel<kBlock>),
-        kBlock, L.n_items);
-    TimedLaunch tl(ctx, KID_SWE_MIXED, s, L.pairs);
-    hipLaunchKernelGGL((swe_ens_mixed_kernel<kBlock>), dim3(grid),
-                       dim3(kBlock), 0, s, static_cast<const SweSrc*>(src),
+  const SweVariant& v = swe_variant(ctx, precision, d.n_verts + d.n_faces);
+  const int grid = ensemble_grid(ctx, reinterpret_cast<const void*>(v.ens),
+                                 kBlock, L.n_items);
+  {
+    TimedLaunch tl(ctx, precision == LPM_FP64 ? KID_SWE_F64 : KID_SWE_MIXED, s,
+                   L.pairs);
+    hipLaunchKernelGGL(v.ens, dim3(grid), dim3(kBlock), 0, s,
+                       static_cast<const SweSrc*>(src),
                        static_cast<const double4*>(bounds), A);
   }
   LPM_HIP_CHECK(hipGetLastError());

@@ -96,6 +96,55 @@ def test_swe_sums_mixed_match_fp64(ctx):
     assert rel_l2(l32[lf], l64[lf]) < 1e-6
 
 
+def _variant_names(ctx, kid):
+    return ctx.variants(kid)
+
+
+@pytest.mark.parametrize("eps_h", [0.0, 1.0])
+def test_every_swe_variant_matches(ctx, ora, eps_h):
+    """Each compiled shape of the sums (reciprocal sharing, occupancy):
+    FP64 against the oracle at 1e-12, mixed against FP64 at 1e-6, on the
+    collocated face sums of a masked cubed-sphere mesh."""
+    m, zf, sf, hf, hv = _fields(1, 4)
+    h = m.mean_spacing()
+    eps, pse = eps_h * h, 2 * h
+    mask = dev(m.face_mask, torch.uint8)
+    fx = dev(m.face_x)
+    leaf = m.face_mask == 0
+    ru, rdd, rlap = ora.swe_sums(aos(m.face_x), hf, aos(m.face_x), zf, sf,
+                                 m.face_area, hf, m.face_mask, collocated=True,
+                                 eps=eps, pse_eps=pse)
+    k64, kmx = _abi.LPM_KERNEL_SWE_F64, _abi.LPM_KERNEL_SWE_MIXED
+    assert len(_variant_names(ctx, k64)) >= 2
+    assert len(_variant_names(ctx, kmx)) >= 2
+    ref64 = None
+    try:
+        for v, name in enumerate(_variant_names(ctx, k64)):
+            ctx.set_variant(k64, v)
+            u, dd, lap = ctx.swe_sums(fx, dev(hf), fx, dev(zf), dev(sf),
+                                      dev(m.face_area), dev(hf), src_mask=mask,
+                                      collocated=True, eps=eps, pse_eps=pse)
+            u, dd, lap = u.cpu().numpy().T, dd.cpu().numpy(), lap.cpu().numpy()
+            assert rel_l2(u[leaf], ru[leaf]) < TOL64, name
+            assert rel_l2(dd[leaf], rdd[leaf]) < TOL64, name
+            assert rel_l2(lap[leaf], rlap[leaf]) < TOL64, name
+            if ref64 is None:
+                ref64 = (u, dd, lap)
+        for v, name in enumerate(_variant_names(ctx, kmx)):
+            ctx.set_variant(kmx, v)
+            u, dd, lap = ctx.swe_sums(fx, dev(hf), fx, dev(zf), dev(sf),
+                                      dev(m.face_area), dev(hf), src_mask=mask,
+                                      collocated=True, eps=eps, pse_eps=pse,
+                                      precision="mixed")
+            u, dd, lap = u.cpu().numpy().T, dd.cpu().numpy(), lap.cpu().numpy()
+            assert rel_l2(u[leaf], ref64[0][leaf]) < 1e-6, name
+            assert rel_l2(dd[leaf], ref64[1][leaf]) < 1e-6, name
+            assert rel_l2(lap[leaf], ref64[2][leaf]) < 1e-6, name
+    finally:
+        ctx.set_variant(k64, -1)
+        ctx.set_variant(kmx, -1)
+
+
 def test_swe_sums_tc2_closed_forms(ctx):
     m = mesh(0, 5)
     h = m.mean_spacing()
