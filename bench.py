@@ -211,16 +211,6 @@ def ensemble_run(ctx, steps, warmup, dt=0.01, autotune=True):
         "graph": True,
         "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
         "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
-        "kernel_ms_per_evaluation": {"mixed": round(m["kernel_ms"], 3),
-                                     "fp64": round(f["kernel_ms"], 3)},
-        "roofline_fp64_sums": {
-            "flops_per_pair": F_SWE,
-            "achieved": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12, 3),
-            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(pairs * F_SWE / (f["kernel_ms"] * 1e-3) / 1e12
-                          / PEAK_FP64_TFLOPS, 4)},
-        "mixed_speedup_vs_fp64_sums": round(f["kernel_ms"] / m["kernel_ms"], 4),
-        "variants": tuned or None,
         "mixed_vs_fp64_t0_vertices_rel_l2_linf": errors(m["u0"], f["u0"]),
         "mixed_vs_fp64_after_steps_vertices_rel_l2_linf": errors(m["uT"], f["uT"]),
         "mixed_vs_fp64_after_steps_positions_rel_l2_linf": errors(m["xT"], f["xT"]),
