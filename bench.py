@@ -674,47 +674,48 @@ def stream_fn_run(ctx, mesh, reps=3, oracle_stride=512):
 
 
 def cpu_baseline(mesh, z_faces, budget_s=12.0):
-    """Time the oracle (reference-faithful loop: every stored face, masked
-    ones skipped) on a bounded sample of vertex targets."""
+    """The oracle (reference-faithful loop: every stored face, masked ones
+    skipped) on the host's cores, in a child process with its OpenMP threads
+    pinned one per physical core: the icosahedral level-4 run in full and a
+    bounded sample of this workload's evaluation (oracle/cpu_baseline.py).
+    Threads: the physical cores of this process's affinity set, at most
+    OMP_NUM_THREADS (the CPU share a one-GPU job is given)."""
+    import subprocess
+    import tempfile
+
     import numpy as np
+    from lpm_amd.mesh import SphereMesh
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    threads = min(16, os.cpu_count() or 1)
-    fx = np.ascontiguousarray(mesh.face_x.T)
-    vx = np.ascontiguousarray(mesh.vert_x.T)
-    n = 256
-    while True:
-        t0 = time.perf_counter()
-        oracle.bve_velocity(vx[:n], fx, z_faces, mesh.face_area, mesh.face_mask,
-                            nthreads=threads)
-        dt = time.perf_counter() - t0
-        if dt > 0.5 or n >= len(vx):
-            break
-        n = min(len(vx), n * 4)
-    # scale the sample to the time budget
-    n2 = int(min(len(vx), max(n, n * budget_s / max(dt, 1e-3))))
-    stride = max(1, len(vx) // n2)
-    sample = vx[::stride][:n2]
-    t0 = time.perf_counter()
-    u = oracle.bve_velocity(sample, fx, z_faces, mesh.face_area, mesh.face_mask,
-                            nthreads=threads)
-    dt = time.perf_counter() - t0
-    useful = len(sample) * mesh.n_leaves
-    iters = len(sample) * mesh.n_faces
-    return {
-        "value": useful / dt / 1e9,
-        "unit": "Gpair-interactions/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"one FP64 velocity evaluation at {len(sample)} of "
-                   f"{mesh.n_verts} vertex targets ("
-                   + ("the first" if stride == 1 else f"every {stride}th")
-                   + ") over all "
-                   f"{mesh.n_faces} stored faces ({mesh.n_leaves} undivided), "
-                   f"reference-faithful loop, {dt:.1f} s; rate counts useful "
-                   f"pairs ({useful:.3e}); loop iterations {iters:.3e}"),
-        "seconds": dt,
-    }, sample, stride, u
+    import cpu_baseline as CB
+
+    host = CB.host_cpu()
+    cap = int(os.environ.get("OMP_NUM_THREADS") or host["physical_cores_in_affinity"])
+    threads = max(1, min(host["physical_cores_in_affinity"], cap))
+    c1 = SphereMesh.build("icos", 4)
+    with tempfile.TemporaryDirectory(prefix="lpm_bench_cpu_") as tmp:
+        meshes = os.path.join(tmp, "meshes.npz")
+        out = os.path.join(tmp, "sample.npz")
+        np.savez(meshes, c1_vert_x=c1.vert_x, c1_face_x=c1.face_x,
+                 c1_face_area=c1.face_area, c1_face_mask=c1.face_mask,
+                 c2_vert_x=mesh.vert_x, c2_face_x=mesh.face_x,
+                 c2_face_area=mesh.face_area, c2_face_mask=mesh.face_mask)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads),
+                   OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        r = subprocess.run(
+            [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"),
+             "--meshes", meshes, "--out", out, "--threads", str(threads),
+             "--budget", str(budget_s)],
+            env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline failed:\n{r.stderr[-2000:]}")
+        cb = json.loads(r.stdout.strip().splitlines()[-1])
+        with np.load(out) as f:
+            u, stride, n = f["u"], int(f["stride"]), int(f["n"])
+    cb["cores_note"] = (f"{threads} OpenMP threads, one per physical core of "
+                        f"this process's affinity set, capped at "
+                        f"OMP_NUM_THREADS={cap} (the CPU share of a one-GPU "
+                        f"job)")
+    return cb, n, stride, u
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
@@ -981,13 +982,15 @@ def main(argv=None):
     if not a.no_cpu_baseline:
         log("CPU baseline (oracle) on a bounded sample")
         z = G.rh54_vorticity(mesh.face_x, 0.1, -0.1)
-        cb, sample, stride, u_cpu = cpu_baseline(mesh, z, a.cpu_budget)
-        result["cpu_baseline"] = {k: v for k, v in cb.items() if k != "seconds"}
+        cb, n_sample, stride, u_cpu = cpu_baseline(mesh, z, a.cpu_budget)
+        result["cpu_baseline"] = cb
         # parity of the GPU FP64 initial velocity with the oracle, same sample
-        gpu_sample = r64["u0_v"][:, ::stride][:, :len(sample)]
+        gpu_sample = r64["u0_v"][:, ::stride][:, :n_sample]
         result["errors_fp64"]["vs_cpu_oracle_sample_rel_l2_linf"] = errors(
             gpu_sample, u_cpu.T)
-        log(f"CPU: {cb['value']:.3f} Gpairs/s on {cb['cores']} threads")
+        log(f"CPU: {cb['value']:.3f} Gpairs/s on {cb['cores']} threads "
+            f"({cb['host']['model']}), level-4 run "
+            f"{cb['small_workload_full']['seconds']:.2f} s")
         # both FP64 sums against an extended-precision evaluation of the
         # same pair formula (long double products, compensated sums): the
         # GPU kernel and the reference-faithful loop round 1 - x.y and the

@@ -31,6 +31,15 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def load(path: str):
+    """Use the oracle library at ``path`` (e.g. a build for the host's own
+    instruction set) instead of the default one."""
+    global _lib, _LIB_PATH
+    _LIB_PATH = path
+    _lib = None
+    return lib()
+
+
 def lib():
     global _lib
     if _lib is None:
