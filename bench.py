@@ -427,6 +427,10 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
                           / PEAK_FP64_TFLOPS, 4)},
         "mixed_speedup_vs_fp64_sums": round(f["kernel_ms"] / m["kernel_ms"], 4),
         "variants": tuned or None,
+        "counters_fp64_default_shape": valu_counters("swe_ens_f64",
+                                                     "share4_occ3", "6x16"),
+        "counters_mixed_default_shape": valu_counters("swe_ens_mixed", "occ4",
+                                                      "6x16"),
         "mixed_vs_fp64_t0": {
             "velocity_rel_l2_linf": errors(m["t0"]["velocity_passive"],
                                            f["t0"]["velocity_passive"]),
@@ -607,6 +611,45 @@ def diagnostics_run(ctx, n=50_000_000, reps=10):
     return res
 
 
+def solid_body_run(ctx, mesh, omega=2 * 3.141592653589793):
+    """Solid-body rotation on the workload mesh (the reference driver's
+    check, examples/bve_rotation.cpp:147-167): zeta = 2 Omega z, exact
+    u = Omega (-y, x, 0) and psi = Omega z; FP64 and mixed velocity and the
+    FP64 stream function from one evaluation each."""
+    import numpy as np
+    import torch
+    from lpm_amd import _abi
+    from lpm_amd import galleries as G
+    from lpm_amd.bve import BVESphere
+
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    s = BVESphere("icos", mesh.depth, ctx=ctx, mesh=mesh)
+    s.init_vorticity(lambda x: G.solid_body_vorticity(x, omega))
+    fx = mesh.face_x[:, mesh.leaf_slice]
+    out = {"workload": (f"solid-body rotation zeta = 2 Omega z, Omega = "
+                        f"{omega:.6f}, icosahedral level {mesh.depth}; exact "
+                        f"u = Omega (-y, x, 0), psi = Omega z")}
+    for prec in ("fp64", "mixed"):
+        s.init_velocity(prec, flags=skip)
+        torch.cuda.synchronize()
+        uv = s.velocity_verts.cpu().numpy()
+        uf = s.velocity_faces.cpu().numpy()[:, mesh.leaf_slice]
+        out[prec] = {
+            "vs_exact_velocity_vertices_rel_l2_linf": errors(
+                uv, G.solid_body_velocity(mesh.vert_x, omega)),
+            "vs_exact_velocity_faces_rel_l2_linf": errors(
+                uf, G.solid_body_velocity(fx, omega))}
+    s.init_stream_fn(flags=skip)
+    torch.cuda.synchronize()
+    out["fp64"]["vs_exact_stream_fn_vertices_rel_l2_linf"] = errors1(
+        s.stream_fn_verts.cpu().numpy(), omega * mesh.vert_x[2])
+    log("solid body: FP64 velocity rel L2 "
+        f"{out['fp64']['vs_exact_velocity_vertices_rel_l2_linf'][0]:.2e}, "
+        f"mixed {out['mixed']['vs_exact_velocity_vertices_rel_l2_linf'][0]:.2e}")
+    del s
+    return out
+
+
 def stream_fn_run(ctx, mesh, reps=3, oracle_stride=512):
     """BVE stream function (BVESphere::init_stream_fn): the FP64 sum of
     s log(1 - x.y) at every vertex and leaf face of the workload mesh, one
@@ -738,6 +781,32 @@ def measured_traffic(level, variant):
     return None
 
 
+COUNTER_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in (
+    "bve_vel_icos_l7_valu_counters.md", "swe_ens_icos_l6x16_valu_counters.md")]
+
+
+def valu_counters(kernel, variant, workload):
+    """Counter figures of one kernel shape and workload from the committed
+    rocprofv3 summaries (lines ``valu_counters: <kernel> <variant>
+    <workload> key=value ...``), or None."""
+    for path in COUNTER_SUMMARIES:
+        try:
+            with open(path) as f:
+                text = f.read()
+        except OSError:
+            continue
+        for m in re.finditer(r"^valu_counters:\s+(\S+)\s+(\S+)\s+(\S+)\s+(.*)$",
+                             text, re.M):
+            if (m.group(1), m.group(2), m.group(3)) == (kernel, variant,
+                                                        workload):
+                out = {"source": os.path.relpath(path, ROOT)}
+                for kv in m.group(4).split():
+                    k, _, v = kv.partition("=")
+                    out[k] = float(v)
+                return out
+    return None
+
+
 def copy_calibration(mib, reps=5):
     """A plain elementwise copy of known size for scaling byte counters."""
     import torch
@@ -755,8 +824,8 @@ def copy_calibration(mib, reps=5):
 
 
 # legs that --only selects (each has a --no-<leg> switch)
-LEGS = ["mixed", "stream-fn", "diagnostics", "ensemble", "precision-study",
-        "swe", "ic2d", "cpu-baseline"]
+LEGS = ["mixed", "solid-body", "stream-fn", "diagnostics", "ensemble",
+        "precision-study", "swe", "ic2d", "cpu-baseline"]
 
 
 def main(argv=None):
@@ -790,6 +859,8 @@ def main(argv=None):
                     help="skip the 16-state shallow-water ensemble")
     ap.add_argument("--no-ic2d", action="store_true",
                     help="skip the Incompressible2D run")
+    ap.add_argument("--no-solid-body", action="store_true",
+                    help="skip the solid-body rotation check")
     ap.add_argument("--no-stream-fn", action="store_true",
                     help="skip the BVE stream-function evaluation")
     ap.add_argument("--no-diagnostics", action="store_true",
@@ -906,10 +977,8 @@ def main(argv=None):
                                     f"nu t = {nu * t_end:.5f} rad",
         },
         "roofline": {
-            # compute-bound at the FP64 peak (78.6 TFLOP/s on MI355X for both
-            # the matrix cores and the vector ALUs; this kernel runs on the
-            # vector ALUs)
-            "bound": "mfma",
+            # bound by FP64 issue on the vector ALUs (78.6 TFLOP/s)
+            "bound": "valu",
             "compute_unit": "FP64 vector ALU",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP64_TFLOPS,
@@ -925,6 +994,21 @@ def main(argv=None):
         "device": {"name": props.name, "cus": props.multi_processor_count},
         "autotune_fp64": r64["autotune"],
     }
+
+    vc = valu_counters("bve_vel_f64", v64, f"{a.level}")
+    if vc is not None:
+        # hardware-counted FP64 flops per pair from the committed profile,
+        # at this run's kernel time
+        counted = vc["fp64_flops_per_pair"] * pairs_eval / (avg_ms * 1e-3) / 1e12
+        result["roofline"]["counters"] = {
+            "source": vc["source"],
+            "valu_busy": vc["valu_busy"],
+            "fp64_flops_per_pair_counted": vc["fp64_flops_per_pair"],
+            "fp64_instr_per_pair_counted": vc["fp64_instr_per_pair"],
+            "counted_tflops": round(counted, 3),
+            "counted_frac": round(counted / PEAK_FP64_TFLOPS, 4),
+            "profiled_avg_launch_ms": vc["avg_ms"],
+        }
 
     if not a.no_mixed:
         log("mixed run")
@@ -953,9 +1037,16 @@ def main(argv=None):
                 errors(rmx["uT_v"], rh54_at(rmx["xT_v"], t_end)),
             "autotune": rmx["autotune"],
         }
+        vm = (rmx["autotune"] or {}).get("variant") or a.variant_mixed
+        vcm = valu_counters("bve_vel_mfma16", vm, f"{a.level}") if vm else None
+        if vcm is not None:
+            result["mixed"]["counters"] = vcm
         log(f"mixed: {mstep*1e3:.1f} ms/step, {mval:.1f} Gpairs/s "
             f"({mval/value:.2f}x FP64)")
 
+    if not a.no_solid_body:
+        log("solid-body rotation")
+        result["solid_body"] = solid_body_run(ctx, mesh)
     if not a.no_stream_fn:
         log("stream function")
         result["stream_fn"] = stream_fn_run(ctx, mesh)

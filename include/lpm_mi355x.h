@@ -203,6 +203,24 @@ int lpm_bve_stream_fn(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                       int64_t n_src, int collocated, int precision,
                       unsigned flags, double* out_psi, void* stream);
 
+/* Stream function and velocity in one pass (the pair formulas of
+ * lpm_bve_stream_fn and lpm_bve_velocity; both outputs required unless
+ * NULL).  Replaces BVEVertexSolve (collocated = 0) and BVEFaceSolve
+ * (collocated = 1, j == i skipped), which run StreamReduceDistinct /
+ * StreamReduceCollocated and VelocityReduceDistinct /
+ * VelocityReduceCollocated as two reductions per target
+ * (src/lpm_bve_sphere_kernels.hpp:84-132, 276-320); here they are the
+ * fused sums of lpm_ic2d_sums with eps = 0 (FP64: the table-driven FP64
+ * log; LPM_MIXED: FP32 pairs with FP64 tile carries).  Other arguments as
+ * lpm_bve_velocity. */
+int lpm_bve_solve(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                  int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                  int64_t src_es, int64_t src_cs, const double* src_zeta,
+                  const double* src_area, const uint8_t* src_mask,
+                  int64_t n_src, int collocated, int precision, unsigned flags,
+                  double* out_u, int64_t out_es, int64_t out_cs,
+                  double* out_psi, void* stream);
+
 /* ---------------------------------------------------------- BVE RK4 ---- */
 /* Fields of BVESphere<Seed> the stepper reads and writes (device memory).
  * Positions/velocities use (es, cs) strides as above. */

@@ -139,6 +139,9 @@ _SIGS = {
     "lpm_bve_velocity": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
                                  c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_dbl,
                                  c_int, c_uint, c_vp, c_i64, c_i64, c_vp]),
+    "lpm_bve_solve": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
+                              c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int,
+                              c_uint, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "lpm_bve_velocity_noalloc": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp,
                                          c_i64, c_i64, c_vp, c_vp, c_vp, c_i64,
                                          c_int, c_dbl, c_int, c_uint, c_vp,

@@ -455,6 +455,21 @@ int lpm_ic2d_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
   });
 }
 
+int lpm_bve_solve(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
+                  int64_t tgt_cs, int64_t n_tgt, const double* src_x,
+                  int64_t src_es, int64_t src_cs, const double* src_zeta,
+                  const double* src_area, const uint8_t* src_mask,
+                  int64_t n_src, int collocated, int precision, unsigned flags,
+                  double* out_u, int64_t out_es, int64_t out_cs,
+                  double* out_psi, void* stream) {
+  // BVEVertexSolve / BVEFaceSolve are the kernel_vals sums without
+  // smoothing: one fused pass gives both of their reductions
+  return lpm_ic2d_sums(ctx, tgt_x, tgt_es, tgt_cs, n_tgt, src_x, src_es,
+                       src_cs, src_zeta, src_area, src_mask, n_src, collocated,
+                       0.0, precision, flags, out_u, out_es, out_cs, out_psi,
+                       stream);
+}
+
 int lpm_ic2d_init_direct_sums(lpm_ctx* ctx, const lpm_ic2d_fields* f,
                               double eps, int precision, unsigned flags,
                               void* stream) {

@@ -105,6 +105,22 @@ class Context:
             stream if stream is not None else current_stream()))
         return out
 
+    def bve_solve(self, tgt_x, src_x, src_zeta, src_area, src_mask=None,
+                  collocated=False, precision="fp64", flags=0, stream=None):
+        """Velocity (3, n) and stream function (n) in one pass
+        (BVEVertexSolve / BVEFaceSolve)."""
+        tp, tes, tcs, nt = crd_args(tgt_x)
+        sp, ses, scs, ns = crd_args(src_x)
+        u = torch.zeros((3, nt), dtype=torch.float64, device=tgt_x.device)
+        psi = torch.zeros(nt, dtype=torch.float64, device=tgt_x.device)
+        self.check(self.L.lpm_bve_solve(
+            self.h, tp, tes, tcs, nt, sp, ses, scs, _ptr(src_zeta),
+            _ptr(src_area), _ptr(src_mask), ns, int(collocated),
+            precision_id(precision), flags, u.data_ptr(), u.stride(1),
+            u.stride(0), psi.data_ptr(),
+            stream if stream is not None else current_stream()))
+        return u, psi
+
     def bve_init_velocity(self, fields, precision="fp64", flags=0, stream=None):
         self.check(self.L.lpm_bve_init_velocity(
             self.h, ctypes.byref(fields), precision_id(precision), flags,

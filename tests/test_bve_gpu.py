@@ -130,6 +130,32 @@ def test_stream_function_matches_oracle(ctx, ora):
     assert rel_l2(psi.cpu().numpy()[leaf], ref[leaf]) < TOL64
 
 
+@pytest.mark.parametrize("seed,depth", [(0, 3), (1, 3)])
+def test_solve_matches_oracle(ctx, ora, seed, depth):
+    """lpm_bve_solve (BVEVertexSolve / BVEFaceSolve): velocity and stream
+    function against the oracle's two separate sums."""
+    m, z = _inputs(seed, depth)
+    args = (dev(m.face_x), dev(z), dev(m.face_area),
+            dev(m.face_mask, torch.uint8))
+    u, psi = ctx.bve_solve(dev(m.vert_x), *args)
+    ref_u = ora.bve_velocity(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask)
+    ref_p = ora.bve_stream_fn(aos(m.vert_x), aos(m.face_x), z, m.face_area,
+                              m.face_mask)
+    assert rel_l2(u.cpu().numpy().T, ref_u) < TOL64
+    assert rel_linf(u.cpu().numpy().T, ref_u) < TOL64
+    assert rel_l2(psi.cpu().numpy(), ref_p) < TOL64
+    u, psi = ctx.bve_solve(dev(m.face_x), *args, collocated=True,
+                           flags=_abi.LPM_FLAG_SKIP_MASKED_TARGETS)
+    ref_u = ora.bve_velocity(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                             m.face_mask, collocated=True)
+    ref_p = ora.bve_stream_fn(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                              m.face_mask, collocated=True)
+    lf = m.face_mask == 0
+    assert rel_l2(u.cpu().numpy().T[lf], ref_u[lf]) < TOL64
+    assert rel_l2(psi.cpu().numpy()[lf], ref_p[lf]) < TOL64
+
+
 def test_stream_function_log_accuracy_and_special_values(ctx):
     # one source at the pole: psi_i = s log(1 - z_i) with s = -zeta A / 4 pi,
     # so the kernel's table-driven FP64 log is compared directly against the
