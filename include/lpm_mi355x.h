@@ -54,7 +54,9 @@ typedef enum lpm_status {
  * about delta / 2.  The time steppers keep positions on the sphere only
  * with LPM_RK4_RENORMALIZE; without it, stage positions leave the sphere by
  * O((dt |u|)^2) and the mixed sums then differ from FP64 by that much
- * (1e-5 relative at dt |u| = 0.007), not by the ~1e-7 of the pair kernel. */
+ * (1e-5 relative at dt |u| = 0.007), not by the ~1e-7 of the pair kernel.
+ * The C++ steppers (lpm/*.hpp) print a warning to stderr when they are
+ * constructed with LPM_MIXED and without LPM_RK4_RENORMALIZE. */
 typedef enum lpm_precision {
   LPM_FP64 = 0,  /* FP64 pair arithmetic, the reference's formula order */
   LPM_MIXED = 1  /* FP32 pair kernel (|x-y|^2/2 form), FP64 carries/state */

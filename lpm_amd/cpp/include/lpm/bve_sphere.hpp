@@ -179,7 +179,9 @@ class BVERK4 {
         nverts(sph.n_vertices_host()),
         nfaces(sph.n_faces_host()),
         precision_(precision),
-        flags_(flags) {}
+        flags_(flags) {
+    warn_if_mixed_off_sphere(precision, flags, "BVERK4");
+  }
 
   template <typename SeedType>
   void advance_timestep(BVESphere<SeedType>& sph) {

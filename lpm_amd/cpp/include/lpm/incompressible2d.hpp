@@ -148,6 +148,7 @@ class Incompressible2DRK2 {
                       unsigned flags = 0, int precision = LPM_FP64)
       : dt(timestep), t_idx(0), flags_(flags), precision_(precision) {
     (void)ic2d;
+    warn_if_mixed_off_sphere(precision, flags, "Incompressible2DRK2");
   }
 
   void advance_timestep_impl(Incompressible2D<SeedType>& ic2d) {

@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,24 @@ inline void check(int rc, const lpm_ctx* ctx = nullptr) {
   if (rc != LPM_OK)
     throw std::runtime_error(std::string("lpm: ") + lpm_last_error(ctx) +
                              " (status " + std::to_string(rc) + ")");
+}
+
+// The mixed-precision pair kernels evaluate 1 - x.y as |x - y|^2 / 2, which
+// holds for positions on the unit sphere only: a stepper that runs them
+// without LPM_RK4_RENORMALIZE lets stage positions leave the sphere by
+// O((dt |u|)^2) and its sums then differ from FP64 by that much.  Steppers
+// print this once per process to stderr.
+inline void warn_if_mixed_off_sphere(int precision, unsigned flags,
+                                     const char* stepper) {
+  static bool warned = false;
+  if (precision == LPM_MIXED && !(flags & LPM_RK4_RENORMALIZE) && !warned) {
+    warned = true;
+    std::fprintf(stderr,
+                 "lpm: %s with LPM_MIXED but without LPM_RK4_RENORMALIZE: the "
+                 "mixed sums assume positions on the unit sphere; add "
+                 "LPM_RK4_RENORMALIZE to the flags\n",
+                 stepper);
+  }
 }
 
 inline void hip_check(hipError_t e, const char* what) {

@@ -174,6 +174,7 @@ class SWERK4 {
          unsigned flags = LPM_RK4_REFERENCE_COMPAT)
       : dt(timestep), t_idx(0), precision_(precision), flags_(flags) {
     (void)swe;
+    warn_if_mixed_off_sphere(precision, flags, "SWERK4");
   }
 
   void advance_timestep_impl(SWE<SeedType>& swe) {
@@ -203,6 +204,7 @@ class SWERK2 {
          unsigned flags = 0)
       : dt(timestep), t_idx(0), precision_(precision), flags_(flags) {
     (void)swe;
+    warn_if_mixed_off_sphere(precision, flags, "SWERK2");
   }
 
   void advance_timestep_impl(SWE<SeedType>& swe) {

@@ -61,6 +61,27 @@ def test_err_norms_mask_error_field_and_strides(ctx, ora):
     np.testing.assert_array_equal(err.cpu().numpy(), masked_appx - exact)
 
 
+def test_err_norms_grid_stride_with_mask_weights_error_field(ctx, ora):
+    """Above 4096 blocks x 256 threads of the partial-sum grid: every thread
+    runs several grid-stride iterations and the final pass reduces 4096
+    partials; with mask, weights, the error output and AoS strides."""
+    rng = np.random.default_rng(11)
+    n = 3_000_017
+    exact = rng.normal(size=(3, n))
+    appx = exact + 1e-2 * rng.normal(size=(3, n))
+    mask = (rng.uniform(size=n) < 0.2).astype(np.uint8)
+    wt = rng.uniform(0.1, 1.0, size=n)
+    a_aos = dev(appx.T).T
+    x_aos = dev(exact.T).T
+    err = torch.full((n, 3), 7.0, dtype=torch.float64, device="cuda").T
+    got = ctx.err_norms(a_aos, x_aos, weight=dev(wt),
+                        mask=dev(mask, torch.uint8), err=err)
+    masked_appx = np.where(mask[None, :] != 0, exact, appx)
+    want = ora.err_norms(masked_appx.T, exact.T, wt)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(err.cpu().numpy(), masked_appx - exact)
+
+
 def test_err_norms_exact_match_and_empty(ctx):
     x = dev(np.ones((3, 10)))
     assert ctx.err_norms(x, x) == (0.0, 0.0, 0.0)

@@ -120,3 +120,28 @@ def test_test_case_2(evaluate):
                  u0max * np.hypot(x[0], x[1]))
     assert np.max(np.abs((o[11:14] * x).sum(axis=0))) < 1e-15
     assert close(o[17], 10 + Omega * (2 * np.pi / 12) * (1 - x[2] ** 2))
+
+
+def test_mixed_without_renormalisation_warns_once(tmp_path):
+    """The C++ steppers' guard: LPM_MIXED without LPM_RK4_RENORMALIZE prints
+    one warning per process to stderr; FP64, or mixed with the flag, none."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "warn.cpp"
+    src.write_text(
+        '#include "lpm/lpm.hpp"\n'
+        "int main() {\n"
+        "  Lpm::warn_if_mixed_off_sphere(LPM_FP64, 0, \"A\");\n"
+        "  Lpm::warn_if_mixed_off_sphere(LPM_MIXED, LPM_RK4_RENORMALIZE, \"B\");\n"
+        "  Lpm::warn_if_mixed_off_sphere(LPM_MIXED, LPM_RK4_REFERENCE_COMPAT, \"C\");\n"
+        "  Lpm::warn_if_mixed_off_sphere(LPM_MIXED, 0, \"D\");\n"
+        "  return 0;\n}\n")
+    exe = str(tmp_path / "warn")
+    r = subprocess.run([HIPCC, "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+                        f"-I{os.path.join(ROOT, 'lpm_amd', 'cpp', 'include')}",
+                        str(src), "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0
+    lines = [l for l in r.stderr.splitlines() if l.startswith("lpm:")]
+    assert len(lines) == 1 and lines[0].startswith("lpm: C with LPM_MIXED")
