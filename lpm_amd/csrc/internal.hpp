@@ -165,6 +165,8 @@ struct lpm_ctx {
 
   lpm_amd::DevBuf swe_packed;   // SWE sources (64 B each)
   lpm_amd::DevBuf swe_bounds;   // SWE tile bounding spheres
+  lpm_amd::DevBuf swe_feat;     // SWE matrix-core tile records
+  lpm_amd::DevBuf swe_far;      // SWE two-pass far sums (18 per target)
 
   // ensemble work plans (keyed by targets per work item) and scratch
   struct EnsemblePlan;
@@ -386,6 +388,11 @@ struct SweTargets {
   double* ddot = nullptr;
   double* laps = nullptr;
 };
+// Allocates the matrix-core workspaces a mixed launch over n_src stored
+// faces and n_targets targets may use (so that a stream capture that follows
+// allocates nothing).
+void swe_feature_workspace(lpm_ctx* ctx, int64_t n_src, int64_t n_targets,
+                           int precision);
 SwePacked swe_pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
                            const double* sigma, const double* area,
                            const double* surf, const RankMap& map,
@@ -395,7 +402,8 @@ void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
                      hipStream_t s);
 // SWE ensembles: workspaces only (returns the plan serial), and the sums of
 // every member at positions (vx, fx) from its own undivided faces.
-uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d);
+uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
+                              int precision);
 void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
                        const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
                        const double* fzeta, const double* fsigma,

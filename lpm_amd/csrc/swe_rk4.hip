@@ -576,6 +576,8 @@ int swe_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt, double Omega,
           const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
           swe_pack_sources(ctx, swe_face_crd(*f), f->face_zeta, f->face_sigma,
                            f->face_area, f->face_surf, map, s);
+          swe_feature_workspace(ctx, f->n_faces, f->n_verts + f->n_faces,
+                                precision);
           return swe_graph_key(*f, dt, Omega, g, eps, pse_eps, precision,
                                flags, order, s, 0);
         },
@@ -605,7 +607,8 @@ int swe_ensemble_step(lpm_ctx* ctx, const lpm_swe_ensemble* e, double dt,
         [&]() {
           build_rank(ctx, f.face_mask, f.n_faces, s);
           return swe_graph_key(f, dt, Omega, g, eps, pse_eps, precision, flags,
-                               order, s, swe_ensemble_prepare(ctx, d));
+                               order, s,
+                               swe_ensemble_prepare(ctx, d, precision));
         },
         [&]() {
           issue_swe_step(ctx, f, dt, Omega, g, flags, order, V, F, ev, s);
