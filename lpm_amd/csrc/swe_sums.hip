@@ -1134,28 +1134,24 @@ __device__ __forceinline__ void swe_mf_far_block(const SweMfTiles& mt,
   }
 }
 
-// Tiles whose flags one round of the second pass keeps in LDS.
-constexpr int kSweFlagChunk = 1024;
-
-// Second pass.  The waves first record, per tile, which of its sub-tiles are
-// near (the first pass's classification: tile near, then sub-tile near; bit
-// 4 w + s for wave w, sub-tile s) and which lie within PSE range of some
-// target of theirs (bit 16 + 4 w + s); the block then stages only flagged
-// tiles into LDS (whole zero-padded tiles) and each wave sums its flagged
-// sub-tiles in FP64.  Then the first pass's far sums are added and the
-// outputs written.  SHARE: pairs per FP64 reciprocal without a self pair.
+// Second pass, each wave on its own (no block barriers, so a wave's near
+// work never idles the others): the wave scans the tiles (the first pass's
+// classification: tile near, then sub-tile near) and the PSE range of its
+// targets, stages each sub-tile it needs into its own LDS slice (one record
+// per lane) and sums it in FP64.  Then the first pass's far sums are added
+// and the outputs written.  SHARE: pairs per FP64 reciprocal without a self
+// pair.  stage: kSweMfSub records of this wave.
 template <int BLOCK, int SHARE>
 __device__ __forceinline__ void swe_mf_near_block(
     const SweSrc* __restrict__ src, const SweMfTiles& mt, int nsrc,
-    const SweSeg& sg, int blk, const SweParams& P, SweSrc* tile64,
-    uint32_t* flags) {
-  static_assert(BLOCK == 256, "four waves per block");
+    const SweSeg& sg, int blk, const SweParams& P, SweSrc* stage) {
   const double eps2 = P.eps2, inv_pse_e2 = P.inv_pse_e2;
   const double pse_cut2 = P.pse_cut2;
   const int do_pse = P.do_pse;
   const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
   const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
-  if (static_cast<int64_t>(blk) * BLOCK >= n_live) return;
+  const int64_t wave0 = c - (threadIdx.x & 63);
+  if (wave0 >= n_live) return;
   const bool in = c < n_live;
   const int64_t i = swe_target(sg, c, in);
   const double* px = sg.x + i * sg.es;
@@ -1163,8 +1159,8 @@ __device__ __forceinline__ void swe_mf_near_block(
   const double s_i = (do_pse && sg.surf) ? sg.surf[i] : 0.0;
   const int self = swe_self(sg, c, i, in, P.skip_self);
   const double rcut = sqrt(pse_cut2);
-  const int wave = threadIdx.x >> 6;
-  const int ntile = (nsrc + kSweTile - 1) / kSweTile;
+  const int lane = threadIdx.x & 63;
+  const int last = max((nsrc + kSweTile - 1) / kSweTile - 1, 0);
   // is x within PSE range of some source in the sphere (c, c.w = r^2)?
   auto pse_near = [&](const double4& s) {
     const double p0 = x[0] - s.x, p1 = x[1] - s.y, p2 = x[2] - s.z;
@@ -1173,59 +1169,42 @@ __device__ __forceinline__ void swe_mf_near_block(
   };
   SweAcc acc;
   acc.zero();
-  for (int t0 = 0; t0 < ntile; t0 += kSweFlagChunk) {
-    const int t1 = min(ntile, t0 + kSweFlagChunk);
-    __syncthreads();  // the previous round's flags and tile are done with
-    for (int t = t0 + threadIdx.x; t < t1; t += BLOCK) flags[t - t0] = 0u;
-    __syncthreads();
-    double4 cc_next = vload(mt.cen, t0);
-    for (int t = t0; t < t1; ++t) {
-      const double4 cc = cc_next;
-      cc_next = vload(mt.cen, min(t + 1, t1 - 1));
-      const bool near = __ballot(in && swe_mf_near(x, cc)) != 0;
-      const bool pse = do_pse && __ballot(pse_near(cc)) != 0;
-      if (!near && !pse) continue;
-      unsigned f = 0u;
-#pragma unroll
-      for (int sub = 0; sub < kSweMfNsub; ++sub) {
-        const double4 cs = mt.scen[kSweMfNsub * static_cast<int64_t>(t) + sub];
-        if (cs.w < 0.0) continue;
-        if (near && __ballot(in && swe_mf_near(x, cs))) f |= 1u << sub;
-        if (pse && __ballot(pse_near(cs))) f |= 16u << sub;
-      }
-      if (f && (threadIdx.x & 63) == 0)
-        atomicOr(&flags[t - t0], ((f & 15u) << (4 * wave)) |
-                                     ((f >> 4) << (16 + 4 * wave)));
-    }
-    __syncthreads();
-    for (int t = t0; t < t1; ++t) {
-      const unsigned f = flags[t - t0];  // block-uniform
-      if (f == 0u) continue;
-      const int j0 = t * kSweTile;
-      tile64[threadIdx.x] = src[j0 + threadIdx.x];
-      __syncthreads();
-      const unsigned mine = (f >> (4 * wave)) & 15u;
-      const unsigned mine_pse = (f >> (16 + 4 * wave)) & 15u;
+  double4 cc_next = vload(mt.cen, 0);
+  for (int t = 0; t <= last && t * kSweTile < nsrc; ++t) {
+    const double4 cc = cc_next;
+    cc_next = vload(mt.cen, min(t + 1, last));
+    const bool near = __ballot(in && swe_mf_near(x, cc)) != 0;
+    const bool pse = do_pse && __ballot(pse_near(cc)) != 0;
+    if (!near && !pse) continue;
 #pragma unroll 1
-      for (int sub = 0; sub < kSweMfNsub; ++sub) {
-        const int s0 = j0 + sub * kSweMfSub;
-        const int cnt = min(kSweMfSub, nsrc - s0);
-        const SweSrc* st = tile64 + sub * kSweMfSub;
-        if (mine & (1u << sub)) {
-          const bool diag = static_cast<unsigned>(self - s0) <
-                            static_cast<unsigned>(cnt);
-          if (__ballot(diag))
-            swe_tile_f64<2>(acc, st, cnt, x, eps2, self - s0, true);
-          else
-            swe_tile_f64<SHARE>(acc, st, cnt, x, eps2, -1, false);
-        }
-        if (mine_pse & (1u << sub)) {
-          double L = 0.0;
-          swe_pse_tile(st, cnt, x, s_i, inv_pse_e2, pse_cut2, L);
-          acc.L1 += L;
-        }
+    for (int sub = 0; sub < kSweMfNsub; ++sub) {
+      const double4 cs = mt.scen[kSweMfNsub * static_cast<int64_t>(t) + sub];
+      if (cs.w < 0.0) continue;
+      const bool nsub = near && __ballot(in && swe_mf_near(x, cs)) != 0;
+      const bool psub = pse && __ballot(pse_near(cs)) != 0;
+      if (!nsub && !psub) continue;
+      const int s0 = t * kSweTile + sub * kSweMfSub;
+      const int cnt = min(kSweMfSub, nsrc - s0);
+      // the previous sub-tile's reads are done before it is overwritten;
+      // the writes land before any lane reads (LDS is in order per wave)
+      __builtin_amdgcn_wave_barrier();
+      stage[lane] = src[s0 + lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (nsub) {
+        const bool diag = static_cast<unsigned>(self - s0) <
+                          static_cast<unsigned>(cnt);
+        if (__ballot(diag))
+          swe_tile_f64<2>(acc, stage, cnt, x, eps2, self - s0, true);
+        else
+          swe_tile_f64<SHARE>(acc, stage, cnt, x, eps2, -1, false);
       }
-      __syncthreads();
+      if (psub) {
+        double L = 0.0;
+        swe_pse_tile(stage, cnt, x, s_i, inv_pse_e2, pse_cut2, L);
+        acc.L1 += L;
+      }
     }
   }
   if (!in) return;
@@ -1261,13 +1240,12 @@ template <int BLOCK, int MINW, int SHARE>
 __global__ __launch_bounds__(BLOCK, MINW) void swe_mf_near_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__,
     const int32_t* __restrict__ count, SweSeg s0, SweSeg s1, SweParams P) {
-  __shared__ SweSrc tile64[kSweTile];
-  __shared__ uint32_t flags[kSweFlagChunk];
+  __shared__ SweSrc stage[BLOCK];
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SweSeg& sg = first ? s0 : s1;
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  swe_mf_near_block<BLOCK, SHARE>(src, P.mf, *count, sg, blk, P, tile64,
-                                  flags);
+  swe_mf_near_block<BLOCK, SHARE>(src, P.mf, *count, sg, blk, P,
+                                  stage + (threadIdx.x & ~63));
 }
 
 // ------------------------------------------------------------ ensembles --
@@ -1447,8 +1425,7 @@ template <int BLOCK, int MINW, int SHARE>
 __global__ __launch_bounds__(BLOCK, MINW) void swe_ens_mf_near_kernel(
     const SweSrc* __restrict__ src, const double4* __restrict__,
     SweEnsArgs A) {
-  __shared__ SweSrc tile64[kSweTile];
-  __shared__ uint32_t flags[kSweFlagChunk];
+  __shared__ SweSrc stage[BLOCK];
   __shared__ int slot;
   for (;;) {
     const int it = swe_ens_next(A, &slot);
@@ -1457,7 +1434,8 @@ __global__ __launch_bounds__(BLOCK, MINW) void swe_ens_mf_near_kernel(
     const EnsMember M = A.mem[w.x];
     const SweSeg sg = swe_ens_segment(A, M, w.y);
     swe_mf_near_block<BLOCK, SHARE>(src + M.P, A.P.mf.from(M.P / kSweTile),
-                                    M.cnt, sg, w.z, A.P, tile64, flags);
+                                    M.cnt, sg, w.z, A.P,
+                                    stage + (threadIdx.x & ~63));
   }
 }
 
@@ -1490,13 +1468,10 @@ const SweVariant kSweMixedVariants[] = {
     // two passes: far tiles and sub-tiles on the matrix cores, then near
     // sub-tiles and PSE terms in FP64 and the outputs
     {swe_mf_far_kernel<kSweBlock, 2, 4>, swe_ens_mf_far_kernel<kSweBlock, 2, 4>,
-     "mfma_f2n4", true, swe_mf_near_kernel<kSweBlock, 4, 4>,
-     swe_ens_mf_near_kernel<kSweBlock, 4, 4>},
-    {swe_mf_far_kernel<kSweBlock, 2, 4>, swe_ens_mf_far_kernel<kSweBlock, 2, 4>,
      "mfma_f2n3", true, swe_mf_near_kernel<kSweBlock, 3, 4>,
      swe_ens_mf_near_kernel<kSweBlock, 3, 4>},
-    {swe_mf_far_kernel<kSweBlock, 3, 2>, swe_ens_mf_far_kernel<kSweBlock, 3, 2>,
-     "mfma_f3n4", true, swe_mf_near_kernel<kSweBlock, 4, 4>,
+    {swe_mf_far_kernel<kSweBlock, 2, 4>, swe_ens_mf_far_kernel<kSweBlock, 2, 4>,
+     "mfma_f2n4", true, swe_mf_near_kernel<kSweBlock, 4, 4>,
      swe_ens_mf_near_kernel<kSweBlock, 4, 4>},
     // FP32 pair arithmetic on the vector ALUs
     {swe_mixed_kernel<kSweBlock, 4>, swe_ens_mixed_kernel<kSweBlock, 4>,
