@@ -413,7 +413,10 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
         "unit": "Gpair-interactions/s",
         "ms_per_step": round(m["elapsed"] / steps * 1e3, 3),
         "rk4_steps_per_s": round(steps / m["elapsed"], 4),
-        "dtype": "f32 pair arithmetic (PSE terms f64), f64 state",
+        "dtype": ("far tiles and 64-source sub-tiles: FP32 distances and "
+                  "f16 hi/lo weight x feature products on the matrix cores "
+                  "(FP32 accumulation, FP64 carry); near sub-tiles and PSE "
+                  "terms f64; f64 state"),
         "graph": True,
         "fp64_value": round(4 * pairs * steps / f["elapsed"] / 1e9, 3),
         "fp64_ms_per_step": round(f["elapsed"] / steps * 1e3, 3),
@@ -429,8 +432,10 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
         "variants": tuned or None,
         "counters_fp64_default_shape": valu_counters("swe_ens_f64",
                                                      "share4_occ3", "6x16"),
-        "counters_mixed_default_shape": valu_counters("swe_ens_mixed", "occ4",
-                                                      "6x16"),
+        "counters_mixed_default_shape": {
+            "far_pass": valu_counters("swe_ens_mf_far", "mfma_f2n3", "6x16"),
+            "near_pass": valu_counters("swe_ens_mf_near", "mfma_f2n3",
+                                       "6x16")},
         "mixed_vs_fp64_t0": {
             "velocity_rel_l2_linf": errors(m["t0"]["velocity_passive"],
                                            f["t0"]["velocity_passive"]),

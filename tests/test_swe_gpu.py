@@ -101,6 +101,45 @@ def _variant_names(ctx, kid):
 
 
 @pytest.mark.parametrize("eps_h", [0.0, 1.0])
+def test_mixed_variants_match_fp64_at_level6(ctx, eps_h):
+    """Every mixed shape against FP64 on the icosahedral mesh of level 6
+    (81,920 sources in 320 tiles, so the matrix-core split into far tiles,
+    far 64-source sub-tiles and FP64 near sub-tiles all occur): vertex
+    targets and collocated face targets (self pairs skipped at eps = 0),
+    velocity, double dot and Laplacian <= 1e-6 relative L2."""
+    m, zf, sf, hf, hv = _fields(0, 6)
+    h = m.mean_spacing()
+    eps, pse = eps_h * h, 2 * h
+    mask = dev(m.face_mask, torch.uint8)
+    fx, vx = dev(m.face_x), dev(m.vert_x)
+    src = (dev(zf), dev(sf), dev(m.face_area), dev(hf))
+    leaf = m.face_mask == 0
+
+    def run(prec):
+        v = ctx.swe_sums(vx, dev(hv), fx, *src, src_mask=mask, eps=eps,
+                         pse_eps=pse, precision=prec)
+        f = ctx.swe_sums(fx, dev(hf), fx, *src, src_mask=mask,
+                         collocated=True, eps=eps, pse_eps=pse,
+                         precision=prec)
+        return ([t.cpu().numpy() for t in v], [t.cpu().numpy() for t in f])
+
+    kmx = _abi.LPM_KERNEL_SWE_MIXED
+    (v64, f64) = run("fp64")
+    names = _variant_names(ctx, kmx)
+    assert any(n.startswith("mfma") for n in names)
+    try:
+        for v, name in enumerate(names):
+            ctx.set_variant(kmx, v)
+            vm, fm = run("mixed")
+            for k, what in enumerate(("velocity", "double dot", "laplacian")):
+                assert rel_l2(vm[k], v64[k]) < 1e-6, (name, "vertices", what)
+                assert rel_l2(fm[k][..., leaf], f64[k][..., leaf]) < 1e-6, (
+                    name, "faces", what)
+    finally:
+        ctx.set_variant(kmx, -1)
+
+
+@pytest.mark.parametrize("eps_h", [0.0, 1.0])
 def test_every_swe_variant_matches(ctx, ora, eps_h):
     """Each compiled shape of the sums (reciprocal sharing, occupancy):
     FP64 against the oracle at 1e-12, mixed against FP64 at 1e-6, on the
