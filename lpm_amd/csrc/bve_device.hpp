@@ -304,11 +304,12 @@ __device__ __forceinline__ const double4& sy_of(const double4* tsy, int j,
 // thread of the block must call it (it synchronises); `tile` is the
 // block's LDS staging array of TILE sources (and `tile_sy` of their s y
 // when SY).  PAIR shares one reciprocal between two sources, QUAD between
-// four.  Each tile is summed into its own partial T, then added to S: two
+// four; QU unrolls the four-source loop (the next group's LDS reads then
+// overlap the current group's arithmetic).  Each tile is summed into its own partial T, then added to S: two
 // levels of summation keep the rounding of S (whose component along x
 // cancels in x cross S) from growing with every source.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false, bool QUAD = false, bool SY = false>
+          bool FUSE = false, bool QUAD = false, bool SY = false, int QU = 1>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
                                           const double4* __restrict__ src_sy,
                                           int nsrc, const SegArgs& sg, int blk,
@@ -356,6 +357,7 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
         }
       }
     } else if (QUAD) {
+#pragma unroll QU
       for (int jj = 0; jj < TILE; jj += 4) {
         const double4 ya = tile[jj], yb = tile[jj + 1];
         const double4 yc = tile[jj + 2], yd = tile[jj + 3];

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(
 // SY: the weights come from the packed s y (src_sy), one multiply less per
 // pair; otherwise src_sy is not read.
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false, bool QUAD = false, bool SY = false>
+          bool FUSE = false, bool QUAD = false, bool SY = false, int QU = 1>
 __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
     const double4* __restrict__ src, const double4* __restrict__ src_sy,
     const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, double eps2) {
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD, SY>(
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD, SY, QU>(
       src, src_sy, *count, sg, blk, eps2, tile, tile_sy);
 }
 
@@ -326,6 +326,10 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true, true>,    \
    bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true, true>, B, T, \
    "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy", true}
+#define LPM_F64_QUAD_SY_U(B, T, NR, QU)                                   \
+  {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true, true, QU>,  \
+   bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true, true, QU>,   \
+   B, T, "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU, true}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -347,6 +351,9 @@ const F64Variant kF64Variants[] = {
     LPM_F64_QUAD(256, 4, 1),            // 12
     LPM_F64_QUAD_SY(256, 2, 1),         // 13: weights from the packed s y
     LPM_F64_QUAD_SY(256, 4, 1),         // 14
+    LPM_F64_QUAD_SY_U(256, 4, 1, 2),    // 15: two groups per iteration
+    LPM_F64_QUAD_SY_U(256, 2, 1, 2),    // 16
+    LPM_F64_QUAD_SY_U(256, 2, 1, 4),    // 17
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
