@@ -330,6 +330,10 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, 256, false, NR, true, true, true, true, QU>,  \
    bve_vel_f64_kernel<B, T, 256, true, NR, true, true, true, true, QU>,   \
    B, T, "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU, true}
+#define LPM_F64_QUAD_SY_TILE(B, T, NR, TL, QU)                                 \
+  {bve_vel_f64_kernel<B, T, TL, false, NR, true, true, true, true, QU>,        \
+   bve_vel_f64_kernel<B, T, TL, true, NR, true, true, true, true, QU>, B, T,   \
+   "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU "_tile" #TL, true}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -354,6 +358,9 @@ const F64Variant kF64Variants[] = {
     LPM_F64_QUAD_SY_U(256, 4, 1, 2),    // 15: two groups per iteration
     LPM_F64_QUAD_SY_U(256, 2, 1, 2),    // 16
     LPM_F64_QUAD_SY_U(256, 2, 1, 4),    // 17
+    LPM_F64_QUAD_SY_TILE(256, 4, 1, 512, 2),   // 18: 512-source LDS tiles
+    LPM_F64_QUAD_SY_TILE(256, 4, 1, 1024, 2),  // 19
+    LPM_F64_QUAD_SY_TILE(256, 2, 1, 512, 2),   // 20
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
