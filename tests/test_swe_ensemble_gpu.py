@@ -105,6 +105,35 @@ def test_sums_equal_members_alone(ctx, precision, skip):
         _check_member(ens, s, m, FIELDS_V, FIELDS_F, leaf_only=skip)
 
 
+def test_mixed_larger_members_equal_members_alone(ctx):
+    """Members large enough (icosahedral levels 4 and 5, cubed sphere level
+    4; up to 80 source tiles) that the mixed sums take every path: far
+    tiles and far 64-source sub-tiles on the matrix cores, FP64 near
+    sub-tiles, with each member's tile records and far-sum slots at its own
+    offsets.  Bit-equal to the members alone, and within 1e-6 of FP64."""
+    members = [(0, 4), (1, 4), (0, 5)]
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    out = {}
+    for precision in ("fp64", "mixed"):
+        ens = _ensemble(ctx, members)
+        ens.init_direct_sums(precision, skip)
+        torch.cuda.synchronize()
+        out[precision] = {n: getattr(ens, n).cpu().numpy()
+                          for n in FIELDS_V + FIELDS_F}
+        if precision == "mixed":
+            for m in range(ens.n_members):
+                s = _single(ctx, ens, m)
+                s.init_direct_sums(precision, skip)
+                torch.cuda.synchronize()
+                _check_member(ens, s, m, FIELDS_V, FIELDS_F, leaf_only=True)
+    leaf = np.concatenate([mesh(s, d).face_mask == 0 for s, d in members])
+    for n in FIELDS_V + FIELDS_F:
+        a, b = out["mixed"][n], out["fp64"][n]
+        if n in FIELDS_F:
+            a, b = a[..., leaf], b[..., leaf]
+        assert rel_l2(a, b) < 1e-6, n
+
+
 def test_fp64_ensemble_matches_oracle(ctx, ora):
     ens = _ensemble(ctx)
     ens.init_direct_sums("fp64", _abi.LPM_FLAG_SKIP_MASKED_TARGETS)
