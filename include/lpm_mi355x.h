@@ -72,8 +72,10 @@ typedef enum lpm_precision {
                                      the unit sphere (FP64) */
 #define LPM_RK4_USE_GRAPH 0x40u   /* capture the step in a hipGraph once and
                                      replay it on later identical calls */
-/* Captured step graphs a context keeps (keyed by fields, dt, Omega, flags
- * and stream); beyond this the least recently replayed one is destroyed. */
+/* Captured step graphs a context keeps (keyed by fields, dt, Omega, flags,
+ * stream and the kernel-variant choice: lpm_set_variant, lpm_tune_load and
+ * an autotune make the next graph step capture again); beyond this the
+ * least recently replayed one is destroyed. */
 #define LPM_GRAPH_CACHE_MAX 16
 
 typedef struct lpm_ctx lpm_ctx;

@@ -426,6 +426,7 @@ int lpm_set_variant(lpm_ctx* ctx, int kernel_id, int variant) {
     LPM_ARG_CHECK(variant >= -1 && variant < variant_count(kernel_id),
                   "variant out of range");
     ctx->forced[kernel_id] = variant;
+    ++ctx->choice_serial;
   });
 }
 
@@ -448,6 +449,7 @@ int lpm_tune_load(lpm_ctx* ctx, const char* path) {
     char key[256];
     int v = 0;
     while (std::fscanf(f, "%255s %d", key, &v) == 2) ctx->tuned[key] = v;
+    ++ctx->choice_serial;
     std::fclose(f);
   });
 }

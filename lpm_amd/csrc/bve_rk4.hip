@@ -337,6 +337,7 @@ void autotune(lpm_ctx* ctx, int family, int64_t n_targets, int repeats,
     }
     restore();
     ctx->tuned[tune_key(ctx, family, n_targets)] = best;
+    ++ctx->choice_serial;
     if (best_variant) *best_variant = best;
     if (best_ms) *best_ms = best_t;
   } catch (...) {
@@ -374,7 +375,7 @@ void run_maybe_graph(lpm_ctx* ctx, bool use_graph, hipStream_t s,
   }
   LPM_ARG_CHECK(s != nullptr, "LPM_RK4_USE_GRAPH needs a non-default stream");
   // the key's prepare step allocates; nothing may allocate while capturing
-  const std::string key = key_fn();
+  const std::string key = key_fn() + '|' + std::to_string(ctx->choice_serial);
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end() &&
       it->second->epoch != g_workspace_epoch.load()) {

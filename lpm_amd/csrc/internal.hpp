@@ -199,6 +199,9 @@ struct lpm_ctx {
   // autotuned kernel variants, keyed by (arch, kernel, size bucket)
   std::map<std::string, int> tuned;
   int forced[lpm_amd::KID_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  // bumped whenever `tuned` or `forced` changes: captured graphs record the
+  // kernels of the choice in force when they were captured
+  uint64_t choice_serial = 0;
 };
 
 namespace lpm_amd {

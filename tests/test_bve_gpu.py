@@ -421,6 +421,41 @@ def test_graph_cache_stays_bounded(ctx):
     assert torch.equal(a.velocity_faces, b.velocity_faces)
 
 
+def test_graph_recaptured_after_variant_change():
+    """A captured step records the kernels of the variant in force when it
+    was captured: after set_variant the next graph step captures again, and
+    graph steps stay bit-equal to eager steps under the same choice."""
+    from lpm_amd.bve import BVERK4
+    from lpm_amd.device import Context
+    c = Context(0)
+    kid = _abi.LPM_KERNEL_BVE_VEL_F64
+    L = _abi.lib()
+    n = ctypes.c_int()
+    sizes = []
+    try:
+        a, b = _sphere(c, 0, 3), _sphere(c, 0, 3)
+        a.init_velocity()
+        b.init_velocity()
+        ra, rb = BVERK4(0.01, a), BVERK4(0.01, b, use_graph=True)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            for v in (-1, 0, len(c.variants(kid)) - 1):
+                c.set_variant(kid, v)
+                for _ in range(2):
+                    ra.advance_timestep(a)
+                    rb.advance_timestep(b)
+                c.check(L.lpm_graph_cache_size(c.h, ctypes.byref(n)))
+                sizes.append(n.value)
+        stream.synchronize()
+        assert sizes == [1, 2, 3]
+        # divided faces hold NaN unless leaf_only is off in both
+        assert _same_bits(a.vert_x, b.vert_x)
+        assert _same_bits(a.face_x, b.face_x)
+        assert _same_bits(a.velocity_verts, b.velocity_verts)
+    finally:
+        c.close()
+
+
 def _same_bits(p, q):
     """Equal values, with NaN where the other has NaN."""
     return bool(((p == q) | (p.isnan() & q.isnan())).all())
