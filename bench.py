@@ -375,7 +375,8 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
             "velocity_passive", "double_dot_passive", "surf_lap_passive",
             "surf_passive", "rel_vort_active", "div_active")}
         r = SWEEnsembleRK4(dt, e, prec, reference_compat=True, leaf_only=True,
-                           use_graph=True, stream=stream.cuda_stream)
+                           use_graph=True, stream=stream.cuda_stream,
+                           renormalize=True)
         for _ in range(warmup):
             e.advance_timestep(r)
         stream.synchronize()
@@ -406,7 +407,8 @@ def swe_ensemble_run(ctx, steps=10, warmup=1, dt=0.005, level=6, members=16,
                      f"each), steady zonal flow u0 = 0.2, g = 1, h0 = 1 (polar "
                      f"depth), Omega = 2 pi, height noise U(-1e-4, 1e-4) "
                      f"(splitmix64 0x4C504D35 + b), eps = {eps:.5f}, PSE "
-                     f"width {pse_eps:.5f}, dt = {dt}"),
+                     f"width {pse_eps:.5f}, dt = {dt}, stage positions "
+                     f"renormalised in both precisions"),
         "pairs_per_evaluation": pairs,
         "steps": steps, "warmup": warmup,
         "value": round(4 * pairs * steps / m["elapsed"] / 1e9, 3),
