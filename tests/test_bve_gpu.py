@@ -217,7 +217,7 @@ def test_edge_cases(ctx, ora):
             if prec == "fp64":
                 # random clouds hold close pairs where 1 - x.y cancels; the
                 # reference formula itself then strays from the exact sum,
-                # so judge against the extended-precision sum and allow the
+                # so compare with the extended-precision sum and allow the
                 # reference formula's own error when comparing to it
                 xp = ora.bve_velocity_xp(aos(x), aos(y), zz, aa)
                 assert rel_l2(u, xp) < TOL64, (ns, prec)

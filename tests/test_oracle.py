@@ -1,4 +1,4 @@
-"""Pins the FP64 oracle (CPU) before it is used to judge the HIP kernels.
+"""Pins the FP64 oracle (CPU) before the HIP kernels are checked against it.
 
 Checks, each chosen so that a plausible slip (sign, index, transposition,
 dropped term) fails at least one:
