@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 # host-only sources: strict IEEE operation order for the mesh generator
-HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": []}
+HOST_SOURCES = {"mesh.cpp": ["-ffp-contract=off"], "abi.cpp": [], "order.cpp": []}
 HIP_SOURCES = ["bve_sums.hip", "ensemble.hip", "bve_ensemble.hip", "bve_rk4.hip", "swe_sums.hip",
                "swe_rk4.hip", "ic2d.hip", "diagnostics.hip", "noalloc.hip"]
 EXAMPLES = ["bve_rotation", "sphere_swe_tc2", "sphere_rh54",

@@ -202,6 +202,17 @@ struct lpm_ctx {
   // bumped whenever `tuned` or `forced` changes: captured graphs record the
   // kernels of the choice in force when they were captured
   uint64_t choice_serial = 0;
+
+  // spatially compact slot orders of target sets (target_order)
+  struct TargetOrder {
+    const double* x = nullptr;
+    int64_t es = 0, cs = 0, n = 0;
+    uint64_t ranges = 0;  // hash of the range offsets
+    uint64_t last_use = 0;
+    lpm_amd::DevBuf perm;  // int32 per slot: target index
+  };
+  std::vector<std::unique_ptr<TargetOrder>> orders;
+  uint64_t order_clock = 0;
 };
 
 namespace lpm_amd {
@@ -403,10 +414,22 @@ SwePacked swe_pack_sources(lpm_ctx* ctx, CrdIn sx, const double* zeta,
 void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
                      int nseg, double eps, double pse_eps, int precision,
                      hipStream_t s);
+// Slot order of n targets at positions x (target indices, int32 on the
+// device) that keeps each wave's targets close together: Morton order of
+// the positions within each range [off[m], off[m + 1]) (one range when off
+// is null).  Computed on the host from the positions at the first call for
+// these buffers (it synchronises the stream; not allowed under capture) and
+// kept in the context (at most kOrderCacheMax orders); later calls reuse it
+// as the positions move, which changes only how targets are grouped.
+constexpr int kOrderCacheMax = 8;
+const int32_t* target_order(lpm_ctx* ctx, CrdIn x, int64_t n,
+                            const int64_t* off, int n_ranges, hipStream_t s);
+
 // SWE ensembles: workspaces only (returns the plan serial), and the sums of
 // every member at positions (vx, fx) from its own undivided faces.
 uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
-                              int precision);
+                              const lpm_swe_fields& f, int precision,
+                              hipStream_t s);
 void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
                        const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
                        const double* fzeta, const double* fsigma,

@@ -1,0 +1,112 @@
+// Spatially compact slot order of target sets (host side).
+//
+// The matrix-core and mixed-precision sums decide per wave (64 targets)
+// which source tiles are near and take an exact path for those; a wave whose
+// targets are spread over the sphere finds nearly every tile near.  Mesh
+// vertices come in refinement order (the 12 seed vertices, then each
+// level's edge midpoints), so the first vertex waves span the whole sphere.
+// Ordering the targets by the Morton (Z-order) key of their position keeps
+// each wave's targets together.  Any order gives the same sums up to
+// rounding: it only regroups targets into waves.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace lpm_amd {
+namespace {
+
+// Spreads the low 20 bits of v to every third bit (bit b -> bit 3 b).
+uint64_t spread3(uint64_t v) {
+  v &= 0xFFFFF;
+  v = (v | (v << 32)) & 0x001F00000000FFFFull;
+  v = (v | (v << 16)) & 0x001F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+// Morton key of a point of [-1, 1]^3: 20 bits per coordinate, interleaved.
+uint64_t morton_key(double x, double y, double z) {
+  auto q = [](double c) -> uint64_t {
+    if (!(c > -1.0)) return 0;  // also NaN
+    if (c >= 1.0) return (1u << 20) - 1;
+    return static_cast<uint64_t>((c + 1.0) * 0.5 * (1u << 20));
+  };
+  return spread3(q(x)) | (spread3(q(y)) << 1) | (spread3(q(z)) << 2);
+}
+
+uint64_t hash_ranges(const int64_t* off, int n_ranges) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a
+  for (int m = 0; off && m <= n_ranges; ++m) {
+    h ^= static_cast<uint64_t>(off[m]);
+    h *= 1099511628211ull;
+  }
+  return h ^ static_cast<uint64_t>(n_ranges);
+}
+
+}  // namespace
+
+const int32_t* target_order(lpm_ctx* ctx, CrdIn x, int64_t n,
+                            const int64_t* off, int n_ranges, hipStream_t s) {
+  if (n <= 0) return nullptr;
+  LPM_ARG_CHECK(n < (int64_t(1) << 31), "target count out of range");
+  const uint64_t h = hash_ranges(off, n_ranges);
+  for (auto& e : ctx->orders)
+    if (e->x == x.base && e->es == x.es && e->cs == x.cs && e->n == n &&
+        e->ranges == h) {
+      e->last_use = ++ctx->order_clock;
+      return e->perm.as<int32_t>();
+    }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  LPM_HIP_CHECK(hipStreamIsCapturing(s, &cap));
+  LPM_ARG_CHECK(cap == hipStreamCaptureStatusNone,
+                "target order must be prepared before stream capture");
+  // positions as they are once the stream's earlier work has finished
+  LPM_HIP_CHECK(hipStreamSynchronize(s));
+  const int64_t span = (n - 1) * x.es + 2 * x.cs + 1;
+  std::vector<double> hx(static_cast<size_t>(span));
+  LPM_HIP_CHECK(hipMemcpy(hx.data(), x.base, sizeof(double) * span,
+                          hipMemcpyDeviceToHost));
+  std::vector<std::pair<uint64_t, int32_t>> key(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    const double* p = hx.data() + i * x.es;
+    key[i] = {morton_key(p[0], p[x.cs], p[2 * x.cs]), static_cast<int32_t>(i)};
+  }
+  // sort within each range; ties keep index order
+  std::vector<int64_t> bounds;
+  if (off && n_ranges > 0) {
+    bounds.assign(off, off + n_ranges + 1);
+  } else {
+    bounds = {0, n};
+  }
+  for (size_t m = 0; m + 1 < bounds.size(); ++m)
+    std::sort(key.begin() + bounds[m], key.begin() + bounds[m + 1]);
+  std::vector<int32_t> perm(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) perm[i] = key[i].second;
+  // bounded cache: retire the least recently used order (a graph that
+  // recorded it is re-captured: the epoch changes)
+  while (ctx->orders.size() >= static_cast<size_t>(kOrderCacheMax)) {
+    auto old = ctx->orders.begin();
+    for (auto it = ctx->orders.begin(); it != ctx->orders.end(); ++it)
+      if ((*it)->last_use < (*old)->last_use) old = it;
+    ctx->orders.erase(old);
+    g_workspace_epoch.fetch_add(1);
+  }
+  auto e = std::make_unique<lpm_ctx::TargetOrder>();
+  e->x = x.base;
+  e->es = x.es;
+  e->cs = x.cs;
+  e->n = n;
+  e->ranges = h;
+  e->last_use = ++ctx->order_clock;
+  int32_t* d = static_cast<int32_t*>(e->perm.ensure(sizeof(int32_t) * n));
+  LPM_HIP_CHECK(hipMemcpy(d, perm.data(), sizeof(int32_t) * n,
+                          hipMemcpyHostToDevice));
+  ctx->orders.push_back(std::move(e));
+  return d;
+}
+
+}  // namespace lpm_amd

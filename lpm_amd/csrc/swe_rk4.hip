@@ -257,6 +257,12 @@ void sums_eval(lpm_ctx* ctx, const lpm_swe_fields& f, CrdIn vx, CrdIn fx,
   SweTargets segs[2];
   segs[0].base.x = vx;
   segs[0].base.n = f.n_verts;
+  // vertex slots in the spatial order of the state positions (the stage
+  // positions of a step are close to them)
+  if (precision == LPM_MIXED)
+    segs[0].base.index =
+        target_order(ctx, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
+                     f.n_verts, nullptr, 1, s);
   segs[0].base.out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
   segs[0].surf = f.vert_surf;
   segs[0].ddot = f.vert_ddot;
@@ -523,6 +529,8 @@ int lpm_swe_sphere_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
       t.base.x = CrdIn{tgt_x, tgt_es, tgt_cs};
       t.base.n = n_tgt;
       t.base.out = CrdOut{out_u, out_es, out_cs};
+      if (precision == LPM_MIXED)
+        t.base.index = target_order(ctx, t.base.x, n_tgt, nullptr, 1, s);
     }
     t.surf = tgt_surf;
     t.ddot = out_ddot;
@@ -578,6 +586,8 @@ int swe_step(lpm_ctx* ctx, const lpm_swe_fields* f, double dt, double Omega,
                            f->face_area, f->face_surf, map, s);
           swe_feature_workspace(ctx, f->n_faces, f->n_verts + f->n_faces,
                                 precision);
+          if (precision == LPM_MIXED)
+            target_order(ctx, swe_vert_crd(*f), f->n_verts, nullptr, 1, s);
           return swe_graph_key(*f, dt, Omega, g, eps, pse_eps, precision,
                                flags, order, s, 0);
         },
@@ -608,7 +618,7 @@ int swe_ensemble_step(lpm_ctx* ctx, const lpm_swe_ensemble* e, double dt,
           build_rank(ctx, f.face_mask, f.n_faces, s);
           return swe_graph_key(f, dt, Omega, g, eps, pse_eps, precision, flags,
                                order, s,
-                               swe_ensemble_prepare(ctx, d, precision));
+                               swe_ensemble_prepare(ctx, d, f, precision, s));
         },
         [&]() {
           issue_swe_step(ctx, f, dt, Omega, g, flags, order, V, F, ev, s);

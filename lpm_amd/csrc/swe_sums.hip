@@ -1360,6 +1360,7 @@ __device__ __forceinline__ SweSeg swe_ens_segment(const SweEnsArgs& A,
     SweSeg sg = A.vert;
     sg.n = M.nv;
     sg.slot_base = M.v0;
+    if (A.vert.index) sg.index = A.vert.index + M.v0;
     sg.far = A.far ? A.far + M.v0 : nullptr;
     sg.far_stride = A.far_stride;
     return sg;
@@ -1644,8 +1645,17 @@ void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
   LPM_HIP_CHECK(hipGetLastError());
 }
 
+// Vertex slot order of the mixed sums: from the members' state positions
+// (also for the stage evaluations of a step, whose positions are close).
+const int32_t* swe_ens_vertex_order(lpm_ctx* ctx, const EnsembleDesc& d,
+                                    const lpm_swe_fields& f, hipStream_t s) {
+  return target_order(ctx, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
+                      d.n_verts, d.voff, d.n_members, s);
+}
+
 uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
-                              int precision) {
+                              const lpm_swe_fields& f, int precision,
+                              hipStream_t s) {
   const EnsLayout L = ensemble_plan(ctx, d, 256);
   const int64_t cap = std::max<int64_t>(d.total_padded_upper, kSweTile);
   ctx->swe_packed.ensure(sizeof(SweSrc) * cap);
@@ -1656,6 +1666,7 @@ uint64_t swe_ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
     if (v.ens2)
       ctx->swe_far.ensure(sizeof(double) * kSweMfRows *
                           std::max<int64_t>(d.n_verts + d.n_faces, 1));
+    swe_ens_vertex_order(ctx, d, f, s);
   }
   return L.serial;
 }
@@ -1667,7 +1678,7 @@ void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
                        double pse_eps, int precision, unsigned flags,
                        hipStream_t s) {
   constexpr int kBlock = 256;
-  swe_ensemble_prepare(ctx, d, precision);
+  swe_ensemble_prepare(ctx, d, f, precision, s);
   const EnsLayout L =
       ensemble_layout(ctx, d, map, f.face_mask, kBlock, s);
   SweSrc* src = ctx->swe_packed.as<SweSrc>();
@@ -1696,6 +1707,7 @@ void swe_ensemble_sums(lpm_ctx* ctx, const EnsembleDesc& d,
   A.vert.ocs = f.vert_u_cs;
   A.vert.out_ddot = f.vert_ddot;
   A.vert.out_laps = f.vert_laps;
+  if (precision == LPM_MIXED) A.vert.index = swe_ens_vertex_order(ctx, d, f, s);
   A.face.x = fx.base;
   A.face.es = fx.es;
   A.face.cs = fx.cs;
