@@ -29,6 +29,7 @@ struct EnsArgs {
   const int32_t* rank;  // compacted rank per face (nullptr: no mask)
   const int32_t* inv;   // face per compacted rank (nullptr: no mask)
   int skip_masked;
+  const int32_t* vorder;  // vertex slot -> vertex (nullptr: identity)
 };
 
 // grid (ceil(max member faces / 256), n_members): packed slot P_m + c holds
@@ -65,6 +66,7 @@ __device__ __forceinline__ SegArgs ens_segment(const EnsArgs& A,
     sg.cs = A.vxcs;
     sg.n = M.nv;
     sg.slot_base = M.v0;
+    if (A.vorder) sg.index = A.vorder + M.v0;
     sg.out = A.vu;
     sg.oes = A.vues;
     sg.ocs = A.vucs;
@@ -296,10 +298,21 @@ void ensure_sources(lpm_ctx* ctx, const EnsembleDesc& d, int precision) {
 
 }  // namespace
 
-uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d, int precision) {
+// Vertex slot order of the mixed sums, from the members' state positions
+// (also used for the stage evaluations of a step, whose positions are close).
+const int32_t* bve_ens_vertex_order(lpm_ctx* ctx, const EnsembleDesc& d,
+                                    const lpm_bve_fields& f, hipStream_t s) {
+  return target_order(ctx, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
+                      d.n_verts, d.voff, d.n_members, s);
+}
+
+uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
+                          const lpm_bve_fields& f, int precision,
+                          hipStream_t s) {
   const EnsVariant& v = pick_ens_variant(ctx, d, precision);
   const EnsLayout L = ensemble_plan(ctx, d, v.block * v.tpl);
   ensure_sources(ctx, d, precision);
+  if (precision == LPM_MIXED) bve_ens_vertex_order(ctx, d, f, s);
   return L.serial;
 }
 
@@ -345,6 +358,7 @@ void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,
   A.rank = map.rank;
   A.inv = map.inv;
   A.skip_masked = (flags & LPM_FLAG_SKIP_MASKED_TARGETS) ? 1 : 0;
+  if (precision == LPM_MIXED) A.vorder = bve_ens_vertex_order(ctx, d, f, s);
   if (A.n_items == 0) return;
   const float4* p32 = nullptr;
   const double4* cen = nullptr;

@@ -156,6 +156,11 @@ void velocity_eval(lpm_ctx* ctx, const lpm_bve_fields& f, CrdIn vx, CrdIn fx,
   segs[0].x = vx;
   segs[0].n = f.n_verts;
   segs[0].out = CrdOut{f.vert_u, f.vert_u_es, f.vert_u_cs};
+  // vertex slots in the spatial order of the state positions (the stage
+  // positions of a step are close to them)
+  if (precision == LPM_MIXED)
+    segs[0].index = target_order(ctx, CrdIn{f.vert_x, f.vert_x_es, f.vert_x_cs},
+                                 f.n_verts, nullptr, 1, s);
   segs[1] = face_targets(fx, map, f.face_mask,
                          (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
                          CrdOut{f.face_u, f.face_u_es, f.face_u_cs}, nullptr);
@@ -517,6 +522,8 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
       const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
       pack_sources(ctx, face_crd(*f), f->face_zeta, f->face_area, map,
                    precision, s);
+      if (precision == LPM_MIXED)
+        target_order(ctx, vert_crd(*f), f->n_verts, nullptr, 1, s);
       return 0;
     };
     run_step(ctx, *f, dt, Omega, precision, flags, s, prepare,
@@ -564,7 +571,7 @@ int lpm_bve_ensemble_rk4_step(lpm_ctx* ctx, const lpm_bve_ensemble* e,
     const lpm_bve_fields& f = e->fields;
     auto prepare = [&]() -> uint64_t {
       build_rank(ctx, f.face_mask, f.n_faces, s);
-      return ensemble_prepare(ctx, d, precision);
+      return ensemble_prepare(ctx, d, f, precision, s);
     };
     run_step(ctx, f, dt, Omega, precision, flags, s, prepare,
              ensemble_eval(ctx, d, f, precision, flags, s));

@@ -378,7 +378,9 @@ const char* ens_variant_name(int family, int v);
 int ens_default_variant(int family);
 // BVE: allocates every workspace an evaluation uses (so that a stream
 // capture that follows allocates nothing); returns the plan serial.
-uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d, int precision);
+uint64_t ensemble_prepare(lpm_ctx* ctx, const EnsembleDesc& d,
+                          const lpm_bve_fields& f, int precision,
+                          hipStream_t s);
 // BVE velocity of every member from positions (vx, fx) and face vorticity
 // fzeta, written to f.vert_u / f.face_u.
 void ensemble_velocity(lpm_ctx* ctx, const EnsembleDesc& d,

@@ -863,7 +863,6 @@ typedef float swe_mf4 __attribute__((ext_vector_type(4)));
 typedef _Float16 swe_mh8 __attribute__((ext_vector_type(8)));
 typedef _Float16 swe_mh2 __attribute__((ext_vector_type(2)));
 typedef float swe_mf2 __attribute__((ext_vector_type(2)));
-typedef unsigned int swe_mu4 __attribute__((ext_vector_type(4)));
 
 // Two FP32 values into packed f16 hi parts and packed f16 lo parts.
 __device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi,
@@ -913,21 +912,6 @@ __device__ __forceinline__ void add_qyq(double* G, const double* Y,
   }
 }
 
-// Buffer descriptor of one tile record of `bytes` bytes at p (uniform across
-// the wave).  Raw buffer loads at offsets past `bytes` return zeros, with no
-// memory access.
-constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
-constexpr int kPastRecord = 0x40000000;  // lane offset beyond every record
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void* p,
-                                                            int bytes) {
-  const uint64_t a = reinterpret_cast<uint64_t>(p);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
-  const uint32_t hi =
-      __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
-  void* u = reinterpret_cast<void*>((static_cast<uint64_t>(hi) << 32) | lo);
-  return __builtin_amdgcn_make_buffer_rsrc(u, 0, bytes, kRsrcWord3);
-}
-
 // Far sums of k-blocks kb0 .. kb1 - 1 of one tile (records ya, fr, fr2, fs
 // in global memory; ya relative to the centroid c of the tile or sub-tile
 // summed) for the wave's groups G0 .. G0 + NG - 1 of 16 targets, carried
@@ -969,34 +953,22 @@ __device__ __forceinline__ void swe_mf_far_tile(
   swe_mf4 S1[NG], S2[NG];
 #pragma unroll
   for (int G = 0; G < NG; ++G) S1[G] = S2[G] = swe_mf4{0.0f, 0.0f, 0.0f, 0.0f};
-  // operands of k-block kb through buffer loads on the tile's records (the
-  // k-block in the scalar offset, the lane's slot in the vector offset): a
-  // lane whose column is past a feature set's rows reads past the record
-  // and gets zeros.  The next block's operands are fetched before the
-  // current one is computed.
-  const __amdgpu_buffer_rsrc_t rya = tile_rsrc(ya, kSweTile * 16);
-  const __amdgpu_buffer_rsrc_t rfr = tile_rsrc(lfr, kSweMfFr * 16);
-  const __amdgpu_buffer_rsrc_t rfr2 = tile_rsrc(lfr2, kSweMfFr2 * 16);
-  const int oa = (col * 4 + g) * 4;  // ya[32 kb + col].g
-  const int o1 = col < kSweMfR1 ? (g * kSweMfR1 + col) * 16 : kPastRecord;
-  const int o2 = col < kSweMfR2 ? (g * kSweMfR2 + col) * 16 : kPastRecord;
+  const float* lyaf = reinterpret_cast<const float*>(ya);
+  const int f1 = min(col, kSweMfR1 - 1), f2 = min(col, kSweMfR2 - 1);
+  // operands of k-block kb; the next block's are fetched before the current
+  // one is computed
   struct KbOps {
     float a0, a1;
-    swe_mu4 u1h, u1l, u2h, u2l;
+    uint4 u1h, u1l, u2h, u2l;
   };
   auto fetch = [&](int kb) {
     KbOps o;
-    o.a0 = __builtin_bit_cast(
-        float, __builtin_amdgcn_raw_buffer_load_b32(rya, oa, kb * 512, 0));
-    o.a1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                         rya, oa + 256, kb * 512, 0));
-    const int s1 = kb * (8 * kSweMfR1 * 16), s2 = kb * (8 * kSweMfR2 * 16);
-    o.u1h = __builtin_amdgcn_raw_buffer_load_b128(rfr, o1, s1, 0);
-    o.u1l = __builtin_amdgcn_raw_buffer_load_b128(rfr, o1 + 4 * kSweMfR1 * 16,
-                                                  s1, 0);
-    o.u2h = __builtin_amdgcn_raw_buffer_load_b128(rfr2, o2, s2, 0);
-    o.u2l = __builtin_amdgcn_raw_buffer_load_b128(rfr2, o2 + 4 * kSweMfR2 * 16,
-                                                  s2, 0);
+    o.a0 = lyaf[(32 * kb + col) * 4 + g];
+    o.a1 = lyaf[(32 * kb + 16 + col) * 4 + g];
+    o.u1h = lfr[((kb * 2 + 0) * 4 + g) * kSweMfR1 + f1];
+    o.u1l = lfr[((kb * 2 + 1) * 4 + g) * kSweMfR1 + f1];
+    o.u2h = lfr2[((kb * 2 + 0) * 4 + g) * kSweMfR2 + f2];
+    o.u2l = lfr2[((kb * 2 + 1) * 4 + g) * kSweMfR2 + f2];
     return o;
   };
   KbOps cur = fetch(kb0);
@@ -1004,10 +976,14 @@ __device__ __forceinline__ void swe_mf_far_tile(
   for (int kb = kb0; kb < kb1; ++kb) {
     const KbOps nxt = fetch(min(kb + 1, kb1 - 1));
     const float a0 = cur.a0, a1 = cur.a1;
-    const swe_mh8 A1h = __builtin_bit_cast(swe_mh8, cur.u1h);
-    const swe_mh8 A1l = __builtin_bit_cast(swe_mh8, cur.u1l);
-    const swe_mh8 A2h = __builtin_bit_cast(swe_mh8, cur.u2h);
-    const swe_mh8 A2l = __builtin_bit_cast(swe_mh8, cur.u2l);
+    uint4 u1h = cur.u1h, u1l = cur.u1l, u2h = cur.u2h, u2l = cur.u2l;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    if (col >= kSweMfR1) u1h = u1l = z;
+    if (col >= kSweMfR2) u2h = u2l = z;
+    const swe_mh8 A1h = __builtin_bit_cast(swe_mh8, u1h);
+    const swe_mh8 A1l = __builtin_bit_cast(swe_mh8, u1l);
+    const swe_mh8 A2h = __builtin_bit_cast(swe_mh8, u2h);
+    const swe_mh8 A2l = __builtin_bit_cast(swe_mh8, u2l);
     cur = nxt;
 #pragma unroll
     for (int G = 0; G < NG; ++G) {
