@@ -387,7 +387,8 @@ int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
  * SWE steps and ensembles) are evaluated in the Morton order of their
  * positions, so that each wave's targets lie close together; the order is
  * computed on the host at the first mixed call for a position buffer (that
- * call synchronises the stream once) and reused while the buffer is. */
+ * call synchronises the stream once) and reused while the buffer is; a
+ * first call made under stream capture keeps the given order. */
 int lpm_swe_sphere_sums(lpm_ctx* ctx, const double* tgt_x, int64_t tgt_es,
                         int64_t tgt_cs, int64_t n_tgt, const double* tgt_surf,
                         const double* src_x, int64_t src_es, int64_t src_cs,

@@ -420,9 +420,10 @@ void launch_swe_sums(lpm_ctx* ctx, const SwePacked& src, SweTargets* segs,
 // device) that keeps each wave's targets close together: Morton order of
 // the positions within each range [off[m], off[m + 1]) (one range when off
 // is null).  Computed on the host from the positions at the first call for
-// these buffers (it synchronises the stream; not allowed under capture) and
-// kept in the context (at most kOrderCacheMax orders); later calls reuse it
-// as the positions move, which changes only how targets are grouped.
+// these buffers (it synchronises the stream) and kept in the context (at
+// most kOrderCacheMax orders); later calls reuse it as the positions move,
+// which changes only how targets are grouped.  Returns nullptr (the given
+// order) for n == 0 and for a first call made while s is being captured.
 constexpr int kOrderCacheMax = 8;
 const int32_t* target_order(lpm_ctx* ctx, CrdIn x, int64_t n,
                             const int64_t* off, int n_ranges, hipStream_t s);

@@ -60,10 +60,11 @@ const int32_t* target_order(lpm_ctx* ctx, CrdIn x, int64_t n,
       e->last_use = ++ctx->order_clock;
       return e->perm.as<int32_t>();
     }
+  // under a capture the caller did not prepare for, keep the given order
+  // (the step functions compute theirs before they capture)
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   LPM_HIP_CHECK(hipStreamIsCapturing(s, &cap));
-  LPM_ARG_CHECK(cap == hipStreamCaptureStatusNone,
-                "target order must be prepared before stream capture");
+  if (cap != hipStreamCaptureStatusNone) return nullptr;
   // positions as they are once the stream's earlier work has finished
   LPM_HIP_CHECK(hipStreamSynchronize(s));
   const int64_t span = (n - 1) * x.es + 2 * x.cs + 1;
