@@ -41,6 +41,9 @@ struct SegArgs {
   double* out_scalar;
   int64_t slot_base;
   int32_t rank_base;
+  // source-split launches: chunk k of slot c adds its three sums at
+  // part[(3 k + comp) * n + c]; a reduction kernel forms x cross S
+  double* part;
 };
 
 __device__ __forceinline__ const SegArgs& pick(const SegArgs& a,
@@ -307,14 +310,17 @@ __device__ __forceinline__ const double4& sy_of(const double4* tsy, int j,
 // four; QU unrolls the four-source loop (the next group's LDS reads then
 // overlap the current group's arithmetic).  Each tile is summed into its own partial T, then added to S: two
 // levels of summation keep the rounding of S (whose component along x
-// cancels in x cross S) from growing with every source.
+// cancels in x cross S) from growing with every source.  KS > 1: the block
+// sums only chunk `chunk` of KS equal runs of source tiles and writes S to
+// sg.part (a third summation level, in chunk order, in the reduction).
 template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
-          bool FUSE = false, bool QUAD = false, bool SY = false, int QU = 1>
+          bool FUSE = false, bool QUAD = false, bool SY = false, int QU = 1,
+          int KS = 1>
 __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
                                           const double4* __restrict__ src_sy,
                                           int nsrc, const SegArgs& sg, int blk,
                                           double eps2, double4* tile,
-                                          double4* tile_sy) {
+                                          double4* tile_sy, int chunk = 0) {
   static_assert(TILE % 8 == 0, "tile must be a multiple of the unroll");
   double x[TPL][3], S[TPL][3];
   int self[TPL];
@@ -323,7 +329,10 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
   if (!load_targets<BLOCK, TPL>(sg, blk, x, self, live, tgt)) return;
 #pragma unroll
   for (int k = 0; k < TPL; ++k) S[k][0] = S[k][1] = S[k][2] = 0.0;
-  for (int j0 = 0; j0 < nsrc; j0 += TILE) {
+  const int ntile = (nsrc + TILE - 1) / TILE;
+  const int jbeg = KS > 1 ? (ntile * chunk) / KS * TILE : 0;
+  const int jend = KS > 1 ? (ntile * (chunk + 1)) / KS * TILE : nsrc;
+  for (int j0 = jbeg; j0 < jend; j0 += TILE) {
     double T[TPL][3];
 #pragma unroll
     for (int k = 0; k < TPL; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0;
@@ -398,7 +407,21 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
       S[k][2] += T[k][2];
     }
   }
-  store_cross<TPL>(sg, x, S, live, tgt);
+  if constexpr (KS > 1) {
+    const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+    const int64_t slot0 = static_cast<int64_t>(blk) * (BLOCK * TPL);
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      const int64_t c = slot0 + k * BLOCK + threadIdx.x;
+      if (c >= n_live) continue;
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp)
+        sg.part[(3 * static_cast<int64_t>(chunk) + comp) * sg.n + c] =
+            S[k][comp];
+    }
+  } else {
+    store_cross<TPL>(sg, x, S, live, tgt);
+  }
 }
 
 // --------------------------------------------------------- mixed block ---

@@ -164,7 +164,7 @@ void velocity_eval(lpm_ctx* ctx, const lpm_bve_fields& f, CrdIn vx, CrdIn fx,
   segs[1] = face_targets(fx, map, f.face_mask,
                          (flags & LPM_FLAG_SKIP_MASKED_TARGETS) != 0,
                          CrdOut{f.face_u, f.face_u_es, f.face_u_cs}, nullptr);
-  launch_bve_velocity(ctx, ps, segs, 2, 0.0, precision, s);
+  launch_bve_velocity(ctx, ps, segs, 2, 0.0, precision, s, -1, true, true);
 }
 
 CrdIn vert_crd(const lpm_bve_fields& f) {
@@ -524,6 +524,10 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
                    precision, s);
       if (precision == LPM_MIXED)
         target_order(ctx, vert_crd(*f), f->n_verts, nullptr, 1, s);
+      const int64_t nt = f->n_verts + f->n_faces;
+      const size_t part =
+          bve_split_bytes(precision, bve_velocity_variant(ctx, precision, nt), nt);
+      if (part) ctx->split_part.ensure(part);
       return 0;
     };
     run_step(ctx, *f, dt, Omega, precision, flags, s, prepare,

@@ -153,6 +153,52 @@ __global__ __launch_bounds__(BLOCK) void bve_vel_f64_kernel(
       src, src_sy, *count, sg, blk, eps2, tile, tile_sy);
 }
 
+// Source-split form: block b * KS + k sums source chunk k of target block
+// b into sg.part; bve_split_reduce_kernel then adds the KS chunks in order
+// and writes x cross S.  Equal-sized work items finer than a whole target
+// block let every CU finish at nearly the same time.
+template <int BLOCK, int TPL, int TILE, bool EPS, int NR, bool PAIR,
+          bool FUSE, bool QUAD, bool SY, int QU, int KS>
+__global__ __launch_bounds__(BLOCK) void bve_vel_f64_split_kernel(
+    const double4* __restrict__ src, const double4* __restrict__ src_sy,
+    const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, double eps2) {
+  __shared__ double4 tile[TILE];
+  __shared__ double4 tile_sy[SY ? TILE : 1];
+  const int b = static_cast<int>(blockIdx.x) / KS;
+  const int chunk = static_cast<int>(blockIdx.x) % KS;
+  const bool first = b < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? b : b - s0.nblocks;
+  f64_block<BLOCK, TPL, TILE, EPS, NR, PAIR, FUSE, QUAD, SY, QU, KS>(
+      src, src_sy, *count, sg, blk, eps2, tile, tile_sy, chunk);
+}
+
+// u = x cross (sum over the ks chunks of S, in chunk order) at every live
+// slot of both segments.
+__global__ __launch_bounds__(256) void bve_split_reduce_kernel(SegArgs s0,
+                                                               SegArgs s1,
+                                                               int ks) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const bool first = g < s0.n;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int64_t c = first ? g : g - s0.n;
+  const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+  if (c >= n_live) return;
+  const int64_t i = sg.index ? sg.index[c] : sg.slot_base + c;
+  if (sg.skip && sg.skip[i]) return;
+  double S[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < ks; ++k)
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+      S[comp] += sg.part[(3 * static_cast<int64_t>(k) + comp) * sg.n + c];
+  const double* p = sg.x + i * sg.es;
+  const double x[1][3] = {{p[0], p[sg.cs], p[2 * sg.cs]}};
+  const double SS[1][3] = {{S[0], S[1], S[2]}};
+  const bool live[1] = {true};
+  const int64_t tgt[1] = {i};
+  store_cross<1>(sg, x, SS, live, tgt);
+}
+
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
 __global__ __launch_bounds__(BLOCK) void bve_vel_mixed_kernel(
     const float4* __restrict__ src, const double4* __restrict__ centers,
@@ -301,6 +347,8 @@ struct F64Variant {
   int block, tpl;
   const char* name;
   bool sy = false;  // reads the packed s y
+  int ksplit = 1;   // > 1: source-split kernel (needs the partial sums
+  int twin = -1;    // workspace); twin: the same shape unsplit
 };
 struct MixedVariant {
   MixedKernel fn, fn_eps;  // vector-ALU kernels (or null)
@@ -334,6 +382,12 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, TL, false, NR, true, true, true, true, QU>,        \
    bve_vel_f64_kernel<B, T, TL, true, NR, true, true, true, true, QU>, B, T,   \
    "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU "_tile" #TL, true}
+#define LPM_F64_QUAD_SY_SPLIT(B, T, NR, TL, QU, KS, TWIN)                       \
+  {bve_vel_f64_split_kernel<B, T, TL, false, NR, true, true, true, true, QU, KS>, \
+   bve_vel_f64_split_kernel<B, T, TL, true, NR, true, true, true, true, QU, KS>,  \
+   B, T,                                                                        \
+   "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU "_tile" #TL "_split" #KS, \
+   true, KS, TWIN}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
    bve_vel_mixed_kernel<B, T, 256, true, PAIR>, nullptr, nullptr, B, T,  \
@@ -361,6 +415,12 @@ const F64Variant kF64Variants[] = {
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 512, 2),   // 18: 512-source LDS tiles
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 1024, 2),  // 19
     LPM_F64_QUAD_SY_TILE(256, 2, 1, 512, 2),   // 20
+    // 21-23: the sources in 8 (or 4) chunks per target block, partial sums
+    // reduced in a second kernel (steps and init_velocity only; other calls
+    // run the twin)
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, 2, 8, 19),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, 2, 8, 18),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, 2, 4, 19),
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
@@ -622,9 +682,18 @@ unsigned bve_variant_layouts(int precision, int variant) {
   return kLayoutVec32;
 }
 
+size_t bve_split_bytes(int precision, int variant, int64_t n_targets) {
+  if (precision != LPM_FP64 || variant < 0 || variant >= kNumF64) return 0;
+  const int ks = kF64Variants[variant].ksplit;
+  if (ks <= 1) return 0;
+  return sizeof(double) * 3 * static_cast<size_t>(ks) *
+         static_cast<size_t>(std::max<int64_t>(n_targets, 1));
+}
+
 void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
-                         hipStream_t s, int variant, bool timed) {
+                         hipStream_t s, int variant, bool timed,
+                         bool allow_split) {
   LPM_ARG_CHECK(nseg >= 1 && nseg <= 2, "one or two target segments");
   TargetSeg empty;
   const TargetSeg& t0 = segs[0];
@@ -637,16 +706,43 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
   if (variant < 0) variant = bve_velocity_variant(ctx, precision, t0.n + t1.n);
   if (precision == LPM_FP64) {
     LPM_ARG_CHECK(variant < kNumF64, "variant out of range");
+    double* part = nullptr;
+    if (kF64Variants[variant].ksplit > 1) {
+      // the partial sums live in a context workspace: a step's prepare
+      // sizes it before capture; a call that may not allocate it (no
+      // permission, or capture with it too small) runs the unsplit twin
+      const size_t need = bve_split_bytes(precision, variant, t0.n + t1.n);
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      LPM_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+      if (allow_split && (ctx->split_part.bytes >= need ||
+                          cs == hipStreamCaptureStatusNone))
+        part = static_cast<double*>(ctx->split_part.ensure(need));
+      else
+        variant = kF64Variants[variant].twin;
+    }
     const F64Variant& v = kF64Variants[variant];
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
     LPM_ARG_CHECK(!v.sy || src.sy, "packed s y missing for the FP64 variant");
     TimedLaunch tl(ctx, timed ? KID_BVE_VEL_F64 : -1, s, pairs);
-    hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn, dim3(grid), dim3(v.block), 0, s,
+    if (part) {
+      a0.part = part;
+      a1.part = part + 3 * static_cast<int64_t>(v.ksplit) * t0.n;
+    }
+    hipLaunchKernelGGL(eps2 > 0 ? v.fn_eps : v.fn,
+                       dim3(static_cast<unsigned>(grid) * v.ksplit),
+                       dim3(v.block), 0, s,
                        static_cast<const double4*>(src.data), src.sy, src.count,
                        a0, a1, eps2);
     LPM_HIP_CHECK(hipGetLastError());
+    if (part) {
+      const int64_t nt = t0.n + t1.n;
+      hipLaunchKernelGGL(bve_split_reduce_kernel,
+                         dim3(static_cast<int>((nt + 255) / 256)), dim3(256),
+                         0, s, a0, a1, v.ksplit);
+      LPM_HIP_CHECK(hipGetLastError());
+    }
   } else if (precision == LPM_MIXED) {
     LPM_ARG_CHECK(variant < kNumMixed, "variant out of range");
     const MixedVariant& v = kMixedVariants[variant];

@@ -657,6 +657,65 @@ def test_matrix_core_variant_on_a_larger_mesh(ctx, seed):
         ctx.set_variant(kid, -1)
 
 
+def test_source_split_variants_match_oracle(ctx, ora):
+    """The source-split FP64 variants (each target block's sources in K
+    chunks, partial sums added in chunk order by a second kernel) run where
+    the context may keep their workspace: init_velocity and the RK4 step.
+    Level 4 has 5120 leaf sources, so the chunks hold zero, one or two
+    source tiles.  Against the oracle to 1e-12, and within 1e-13 of the
+    unsplit twin (same shape, one summation level less), which the one-shot
+    velocity entry point runs in their place."""
+    from lpm_amd.bve import BVERK4
+    kid = _abi.LPM_KERNEL_BVE_VEL_F64
+    names = ctx.variants(kid)
+    split = [v for v, n in enumerate(names) if "_split" in n]
+    assert split, names
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    try:
+        for v in split:
+            twin = names.index(names[v].rsplit("_split", 1)[0])
+            for flags in (0, skip):
+                ctx.set_variant(kid, twin)
+                t = _sphere(ctx, 0, 4)
+                t.init_velocity(flags=flags)
+                ctx.set_variant(kid, v)
+                s = _sphere(ctx, 0, 4)
+                s.init_velocity(flags=flags)
+                # the one-shot entry point runs the twin: bit-equal to it
+                u1 = ctx.bve_velocity(s.vert_x, s.face_x, s.rel_vort_faces,
+                                      s.face_area, s.face_mask)
+                assert torch.equal(u1, t.velocity_verts), names[v]
+                st = _oracle_state(s)
+                m = s.mesh
+                ref_v = ora.bve_velocity(st["vx"], st["fx"], st["fzeta"],
+                                         st["farea"], st["fmask"])
+                ref_f = ora.bve_velocity(st["fx"], st["fx"], st["fzeta"],
+                                         st["farea"], st["fmask"],
+                                         collocated=True)
+                leaf = m.face_mask == 0
+                assert rel_l2(st["vu"], ref_v) < TOL64, names[v]
+                assert rel_l2(st["fu"][leaf], ref_f[leaf]) < TOL64, names[v]
+                assert rel_l2(s.velocity_verts.cpu().numpy(),
+                              t.velocity_verts.cpu().numpy()) < 1e-13, names[v]
+                if flags:
+                    # masked faces untouched (their initial zeros)
+                    assert not s.velocity_faces.cpu().numpy()[:, ~leaf].any()
+            s = _sphere(ctx, 1, 4)
+            s.init_velocity()
+            st = _oracle_state(s)
+            r = BVERK4(0.01, s, reference_compat=True)
+            for _ in range(2):
+                r.advance_timestep(s)
+                ora.bve_rk4_step(st, 0.01, OMEGA, compat=True)
+            torch.cuda.synchronize()
+            assert rel_l2(aos(s.vert_x.cpu().numpy()), st["vx"]) < TOL64
+            assert rel_l2(s.rel_vort_faces.cpu().numpy(), st["fzeta"]) < TOL64
+            assert rel_l2(aos(s.velocity_verts.cpu().numpy()),
+                          st["vu"]) < TOL64, names[v]
+    finally:
+        ctx.set_variant(kid, -1)
+
+
 def test_regularised_kernel_matches_pair_formula(ctx, ora):
     # eps > 0: the pair term is kzeta_sphere with eps (the BVE kernel with
     # 1 - x.y + eps^2 in the denominator)
