@@ -992,7 +992,10 @@ def main(argv=None):
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
             "traffic": measured_traffic(a.level, v64),
-            "kernel": "bve_vel_f64_kernel",
+            # a source-split variant is two launches (partial sums, then
+            # their reduction); avg_launch_ms times both
+            "kernel": ("bve_vel_f64_split_kernel + bve_split_reduce_kernel"
+                       if "_split" in v64 else "bve_vel_f64_kernel"),
             "variant": v64,
             "avg_launch_ms": round(avg_ms, 4),
             "flops_per_pair": F_BVE,

@@ -293,6 +293,26 @@ __device__ __forceinline__ void store_cross(const SegArgs& sg,
   }
 }
 
+// Source-split launches: the sums S of chunk `chunk` at slot c go to
+// sg.part[(3 chunk + comp) n + c] for every slot below the live count;
+// bve_split_reduce_kernel adds the chunks and forms x cross S.
+template <int BLOCK, int TPL>
+__device__ __forceinline__ void store_partial(const SegArgs& sg, int blk,
+                                              int chunk,
+                                              const double (*S)[3]) {
+  const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+  const int64_t slot0 = static_cast<int64_t>(blk) * (BLOCK * TPL);
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int64_t c = slot0 + k * BLOCK + threadIdx.x;
+    if (c >= n_live) continue;
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+      sg.part[(3 * static_cast<int64_t>(chunk) + comp) * sg.n + c] =
+          S[k][comp];
+  }
+}
+
 // The packed s y of source j (SY), or any source record the caller passes
 // where acc_w does not read it.
 template <bool SY>
@@ -407,21 +427,10 @@ __device__ __forceinline__ void f64_block(const double4* __restrict__ src,
       S[k][2] += T[k][2];
     }
   }
-  if constexpr (KS > 1) {
-    const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
-    const int64_t slot0 = static_cast<int64_t>(blk) * (BLOCK * TPL);
-#pragma unroll
-    for (int k = 0; k < TPL; ++k) {
-      const int64_t c = slot0 + k * BLOCK + threadIdx.x;
-      if (c >= n_live) continue;
-#pragma unroll
-      for (int comp = 0; comp < 3; ++comp)
-        sg.part[(3 * static_cast<int64_t>(chunk) + comp) * sg.n + c] =
-            S[k][comp];
-    }
-  } else {
+  if constexpr (KS > 1)
+    store_partial<BLOCK, TPL>(sg, blk, chunk, S);
+  else
     store_cross<TPL>(sg, x, S, live, tgt);
-  }
 }
 
 // --------------------------------------------------------- mixed block ---
@@ -965,12 +974,14 @@ __device__ __forceinline__ void mfma16_reduce(mf4 (*D2)[NACC], int h, float* o) 
 }
 
 // NACC: independent weight-product accumulators per target block (1 or 2;
-// two shorten the dependent MFMA chain).
-template <bool EPS, int NACC = 1>
+// two shorten the dependent MFMA chain).  KS > 1: only source chunk `chunk`
+// of KS equal runs of packing tiles, sums to sg.part (as f64_block).
+template <bool EPS, int NACC = 1, int KS = 1>
 __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
                                              const SegArgs& sg, int blk,
                                              float eps2x2, float4* ly,
-                                             float* ls, uint4* la, float4* lt) {
+                                             float* ls, uint4* la, float4* lt,
+                                             int chunk = 0) {
   double x[1][3];
   int self[1];
   bool live[1];
@@ -981,7 +992,10 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
   const int col = lane & 31;  // target column
   const int c4 = lane & 3;
   double S0 = 0.0, S1 = 0.0, S2 = 0.0;
-  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+  const int ntile = (nsrc + kPackTile - 1) / kPackTile;
+  const int jbeg = KS > 1 ? (ntile * chunk) / KS * kPackTile : 0;
+  const int jend = KS > 1 ? (ntile * (chunk + 1)) / KS * kPackTile : nsrc;
+  for (int j0 = jbeg; j0 < jend; j0 += kPackTile) {
     const int t = j0 / kPackTile;
     __syncthreads();
     {
@@ -1025,7 +1039,10 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
     S2 += static_cast<double>(o[2]) + c.z * W;
   }
   double S[1][3] = {{S0, S1, S2}};
-  store_cross<1>(sg, x, S, live, tgt);
+  if constexpr (KS > 1)
+    store_partial<256, 1>(sg, blk, chunk, S);
+  else
+    store_cross<1>(sg, x, S, live, tgt);
 }
 
 }  // namespace

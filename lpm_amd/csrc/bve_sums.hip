@@ -240,6 +240,26 @@ __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
   mfma16_block<EPS, NACC>(src, *count, sg, blk, eps2x2, ly, ls, la, lt);
 }
 
+// Source-split form of bve_vel_mfma16_kernel (block b * KS + k: chunk k of
+// target block b, partial sums to sg.part; bve_split_reduce_kernel adds
+// them): enough equal work items that the last round of waves is full.
+template <bool EPS, int MINW, int NACC, int KS>
+__global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_split_kernel(
+    Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
+    float eps2x2) {
+  __shared__ float4 ly[kPackTile];
+  __shared__ float ls[kPackTile];
+  __shared__ uint4 la[2 * kPackTile];
+  __shared__ float4 lt[kPackTile];
+  const int b = static_cast<int>(blockIdx.x) / KS;
+  const int chunk = static_cast<int>(blockIdx.x) % KS;
+  const bool first = b < s0.nblocks;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int blk = first ? b : b - s0.nblocks;
+  mfma16_block<EPS, NACC, KS>(src, *count, sg, blk, eps2x2, ly, ls, la, lt,
+                              chunk);
+}
+
 // ------------------------------------------------------ stream function ---
 // psi_i = sum_j s_j log(1 - x_i.y_j) with the table-driven FP64 log.
 // CHECK drops the pair jj == self (tile-relative); only tiles holding some
@@ -356,6 +376,8 @@ struct MixedVariant {
   int block, tpl;
   const char* name;
   Mfma16Kernel m16, m16_eps;  // f16-split matrix-core kernels (or null)
+  int ksplit = 1;  // > 1: source-split kernel (partial sums workspace);
+  int twin = -1;   // twin: the same shape unsplit
 };
 
 #define LPM_F64_VARIANT(B, T, NR, PAIR)                                  \
@@ -382,11 +404,11 @@ struct MixedVariant {
   {bve_vel_f64_kernel<B, T, TL, false, NR, true, true, true, true, QU>,        \
    bve_vel_f64_kernel<B, T, TL, true, NR, true, true, true, true, QU>, B, T,   \
    "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU "_tile" #TL, true}
-#define LPM_F64_QUAD_SY_SPLIT(B, T, NR, TL, QU, KS, TWIN)                       \
+// named as the twin (TAG: its tile suffix) + "_split" K
+#define LPM_F64_QUAD_SY_SPLIT(B, T, NR, TL, TAG, QU, KS, TWIN)                  \
   {bve_vel_f64_split_kernel<B, T, TL, false, NR, true, true, true, true, QU, KS>, \
    bve_vel_f64_split_kernel<B, T, TL, true, NR, true, true, true, true, QU, KS>,  \
-   B, T,                                                                        \
-   "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU "_tile" #TL "_split" #KS, \
+   B, T, "block" #B "_tpl" #T "_nr" #NR "_quad_fused_sy_u" #QU TAG "_split" #KS, \
    true, KS, TWIN}
 #define LPM_MIXED_VARIANT(B, T, PAIR)                                     \
   {bve_vel_mixed_kernel<B, T, 256, false, PAIR>,                         \
@@ -415,12 +437,13 @@ const F64Variant kF64Variants[] = {
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 512, 2),   // 18: 512-source LDS tiles
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 1024, 2),  // 19
     LPM_F64_QUAD_SY_TILE(256, 2, 1, 512, 2),   // 20
-    // 21-23: the sources in 8 (or 4) chunks per target block, partial sums
-    // reduced in a second kernel (steps and init_velocity only; other calls
-    // run the twin)
-    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, 2, 8, 19),
-    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, 2, 8, 18),
-    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, 2, 4, 19),
+    // 21-24: the sources in 8 (or 16) chunks per target block, partial
+    // sums reduced in a second kernel (steps and init_velocity only; other
+    // calls run the twin)
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, "_tile1024", 2, 8, 19),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, "_tile512", 2, 8, 18),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, "_tile512", 2, 16, 18),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 256, "", 2, 8, 15),
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
@@ -439,6 +462,16 @@ const MixedVariant kMixedVariants[] = {
     {nullptr, nullptr, nullptr, nullptr, 256, 1,
      "mfma16_occ5_acc2_block256_near3", bve_vel_mfma16_kernel<false, 5, 2>,
      bve_vel_mfma16_kernel<true, 5, 2>},                                 // 10
+    // 11-12: variant 10 with the sources in 2 (4) chunks per target block
+    // (steps and init_velocity only; other calls run variant 10)
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_block256_near3_split2",
+     bve_vel_mfma16_split_kernel<false, 5, 2, 2>,
+     bve_vel_mfma16_split_kernel<true, 5, 2, 2>, 2, 10},                // 11
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_block256_near3_split4",
+     bve_vel_mfma16_split_kernel<false, 5, 2, 4>,
+     bve_vel_mfma16_split_kernel<true, 5, 2, 4>, 4, 10},                // 12
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
@@ -682,13 +715,45 @@ unsigned bve_variant_layouts(int precision, int variant) {
   return kLayoutVec32;
 }
 
+namespace {
+
+// Chunks per target block of a variant (1: not source-split) and its
+// unsplit twin.
+int split_of(int precision, int variant, int* twin) {
+  if (precision == LPM_FP64 && variant >= 0 && variant < kNumF64) {
+    *twin = kF64Variants[variant].twin;
+    return kF64Variants[variant].ksplit;
+  }
+  if (precision == LPM_MIXED && variant >= 0 && variant < kNumMixed) {
+    *twin = kMixedVariants[variant].twin;
+    return kMixedVariants[variant].ksplit;
+  }
+  *twin = -1;
+  return 1;
+}
+
+}  // namespace
+
 size_t bve_split_bytes(int precision, int variant, int64_t n_targets) {
-  if (precision != LPM_FP64 || variant < 0 || variant >= kNumF64) return 0;
-  const int ks = kF64Variants[variant].ksplit;
+  int twin;
+  const int ks = split_of(precision, variant, &twin);
   if (ks <= 1) return 0;
   return sizeof(double) * 3 * static_cast<size_t>(ks) *
          static_cast<size_t>(std::max<int64_t>(n_targets, 1));
 }
+
+namespace {
+
+void launch_split_reduce(const SegArgs& a0, const SegArgs& a1, int ks,
+                         hipStream_t s) {
+  const int64_t nt = a0.n + a1.n;
+  hipLaunchKernelGGL(bve_split_reduce_kernel,
+                     dim3(static_cast<unsigned>((nt + 255) / 256)), dim3(256),
+                     0, s, a0, a1, ks);
+  LPM_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
 
 void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
@@ -704,27 +769,31 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
   if (t0.n + t1.n == 0) return;
   const double pairs = count_pairs(segs, nseg, src.capacity);
   if (variant < 0) variant = bve_velocity_variant(ctx, precision, t0.n + t1.n);
+  LPM_ARG_CHECK(variant >= 0 && variant < (precision == LPM_FP64 ? kNumF64
+                                                                  : kNumMixed),
+                "variant out of range");
+  double* part = nullptr;
+  int twin = -1;
+  if (split_of(precision, variant, &twin) > 1) {
+    // the partial sums live in a context workspace: a step's prepare sizes
+    // it before capture; a call that may not allocate it (no permission, or
+    // capture with it too small) runs the unsplit twin
+    const size_t need = bve_split_bytes(precision, variant, t0.n + t1.n);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LPM_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+    if (allow_split && (ctx->split_part.bytes >= need ||
+                        cs == hipStreamCaptureStatusNone))
+      part = static_cast<double*>(ctx->split_part.ensure(need));
+    else
+      variant = twin;
+  }
   if (precision == LPM_FP64) {
-    LPM_ARG_CHECK(variant < kNumF64, "variant out of range");
-    double* part = nullptr;
-    if (kF64Variants[variant].ksplit > 1) {
-      // the partial sums live in a context workspace: a step's prepare
-      // sizes it before capture; a call that may not allocate it (no
-      // permission, or capture with it too small) runs the unsplit twin
-      const size_t need = bve_split_bytes(precision, variant, t0.n + t1.n);
-      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-      LPM_HIP_CHECK(hipStreamIsCapturing(s, &cs));
-      if (allow_split && (ctx->split_part.bytes >= need ||
-                          cs == hipStreamCaptureStatusNone))
-        part = static_cast<double*>(ctx->split_part.ensure(need));
-      else
-        variant = kF64Variants[variant].twin;
-    }
     const F64Variant& v = kF64Variants[variant];
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
     const int grid = a0.nblocks + a1.nblocks;
     LPM_ARG_CHECK(!v.sy || src.sy, "packed s y missing for the FP64 variant");
+    LPM_ARG_CHECK(v.ksplit == 1 || part, "split workspace missing");
     TimedLaunch tl(ctx, timed ? KID_BVE_VEL_F64 : -1, s, pairs);
     if (part) {
       a0.part = part;
@@ -736,15 +805,8 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                        static_cast<const double4*>(src.data), src.sy, src.count,
                        a0, a1, eps2);
     LPM_HIP_CHECK(hipGetLastError());
-    if (part) {
-      const int64_t nt = t0.n + t1.n;
-      hipLaunchKernelGGL(bve_split_reduce_kernel,
-                         dim3(static_cast<int>((nt + 255) / 256)), dim3(256),
-                         0, s, a0, a1, v.ksplit);
-      LPM_HIP_CHECK(hipGetLastError());
-    }
+    if (part) launch_split_reduce(a0, a1, v.ksplit, s);
   } else if (precision == LPM_MIXED) {
-    LPM_ARG_CHECK(variant < kNumMixed, "variant out of range");
     const MixedVariant& v = kMixedVariants[variant];
     const int per = v.block * v.tpl;
     SegArgs a0 = to_args(t0, per), a1 = to_args(t1, per);
@@ -752,10 +814,18 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
     TimedLaunch tl(ctx, timed ? KID_BVE_VEL_MIXED : -1, s, pairs);
     if (v.m16) {
       LPM_ARG_CHECK(src.mf_a, "matrix-core layout not packed");
+      LPM_ARG_CHECK(v.ksplit == 1 || part, "split workspace missing");
       Mfma16Src m{src.mf_y, src.mf_s, src.mf_a, src.mf_t32, src.mf_c, src.mf_is};
-      hipLaunchKernelGGL(eps2 > 0 ? v.m16_eps : v.m16, dim3(grid),
+      if (part) {
+        a0.part = part;
+        a1.part = part + 3 * static_cast<int64_t>(v.ksplit) * t0.n;
+      }
+      hipLaunchKernelGGL(eps2 > 0 ? v.m16_eps : v.m16,
+                         dim3(static_cast<unsigned>(grid) * v.ksplit),
                          dim3(v.block), 0, s, m, src.count, a0, a1,
                          static_cast<float>(2.0 * eps2));
+      LPM_HIP_CHECK(hipGetLastError());
+      if (part) launch_split_reduce(a0, a1, v.ksplit, s);
     } else if (v.mfn) {
       LPM_ARG_CHECK(src.mf_y, "matrix-core layout not packed");
       MfmaSrc m{src.mf_y, src.mf_q, src.mf_s, src.mf_t, src.mf_c};
