@@ -490,7 +490,13 @@ int lpm_swe_ensemble_rk2_step(lpm_ctx* ctx, const lpm_swe_ensemble* e,
 /* ---------------------------------------------------- kernel variants ---- */
 /* The direct-sum kernels are compiled in several tile/block/unroll shapes
  * ("variants").  No reference counterpart (Kokkos picks TeamPolicy(n, AUTO),
- * src/lpm_bve_rk4_impl.hpp:70-71).  kernel_id is LPM_KERNEL_*. */
+ * src/lpm_bve_rk4_impl.hpp:70-71).  kernel_id is LPM_KERNEL_*.
+ * BVE velocity variants named *_splitK sum each target block's sources in
+ * K chunks and add the partial sums in a second kernel (same results to
+ * summation order); they keep a workspace of 3 K doubles per target in the
+ * context.  lpm_bve_init_velocity and lpm_bve_rk4_step run them (the step
+ * sizes the workspace before any graph capture); lpm_bve_velocity and
+ * lpm_bve_velocity_noalloc run the unsplit variant of the same shape. */
 int lpm_variant_count(int kernel_id);
 const char* lpm_variant_name(int kernel_id, int variant);
 /* The fixed variant used when nothing was tuned or forced. */

@@ -437,13 +437,14 @@ const F64Variant kF64Variants[] = {
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 512, 2),   // 18: 512-source LDS tiles
     LPM_F64_QUAD_SY_TILE(256, 4, 1, 1024, 2),  // 19
     LPM_F64_QUAD_SY_TILE(256, 2, 1, 512, 2),   // 20
-    // 21-24: the sources in 8 (or 16) chunks per target block, partial
+    // 21-25: the sources in 8 to 32 chunks per target block, partial
     // sums reduced in a second kernel (steps and init_velocity only; other
     // calls run the twin)
     LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 1024, "_tile1024", 2, 8, 19),
     LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, "_tile512", 2, 8, 18),
     LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, "_tile512", 2, 16, 18),
     LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 256, "", 2, 8, 15),
+    LPM_F64_QUAD_SY_SPLIT(256, 4, 1, 512, "_tile512", 2, 32, 18),  // 25
 };
 const MixedVariant kMixedVariants[] = {
     LPM_MIXED_VARIANT(256, 2, false),  // 0
@@ -462,8 +463,8 @@ const MixedVariant kMixedVariants[] = {
     {nullptr, nullptr, nullptr, nullptr, 256, 1,
      "mfma16_occ5_acc2_block256_near3", bve_vel_mfma16_kernel<false, 5, 2>,
      bve_vel_mfma16_kernel<true, 5, 2>},                                 // 10
-    // 11-12: variant 10 with the sources in 2 (4) chunks per target block
-    // (steps and init_velocity only; other calls run variant 10)
+    // 11-13: variant 10 with the sources in 2 to 8 chunks per target
+    // block (steps and init_velocity only; other calls run variant 10)
     {nullptr, nullptr, nullptr, nullptr, 256, 1,
      "mfma16_occ5_acc2_block256_near3_split2",
      bve_vel_mfma16_split_kernel<false, 5, 2, 2>,
@@ -472,6 +473,10 @@ const MixedVariant kMixedVariants[] = {
      "mfma16_occ5_acc2_block256_near3_split4",
      bve_vel_mfma16_split_kernel<false, 5, 2, 4>,
      bve_vel_mfma16_split_kernel<true, 5, 2, 4>, 4, 10},                // 12
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_block256_near3_split8",
+     bve_vel_mfma16_split_kernel<false, 5, 2, 8>,
+     bve_vel_mfma16_split_kernel<true, 5, 2, 8>, 8, 10},                // 13
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
