@@ -42,8 +42,9 @@ struct SegArgs {
   int64_t slot_base;
   int32_t rank_base;
   // source-split launches: chunk k of slot c adds its three sums at
-  // part[(3 k + comp) * n + c]; a reduction kernel forms x cross S
+  // part[(3 k + comp) * pstride + c]; a reduction kernel forms x cross S
   double* part;
+  int64_t pstride;
 };
 
 __device__ __forceinline__ const SegArgs& pick(const SegArgs& a,
@@ -294,8 +295,8 @@ __device__ __forceinline__ void store_cross(const SegArgs& sg,
 }
 
 // Source-split launches: the sums S of chunk `chunk` at slot c go to
-// sg.part[(3 chunk + comp) n + c] for every slot below the live count;
-// bve_split_reduce_kernel adds the chunks and forms x cross S.
+// sg.part[(3 chunk + comp) pstride + c] for every slot below the live
+// count; split_reduce_one adds the chunks and forms x cross S.
 template <int BLOCK, int TPL>
 __device__ __forceinline__ void store_partial(const SegArgs& sg, int blk,
                                               int chunk,
@@ -308,9 +309,28 @@ __device__ __forceinline__ void store_partial(const SegArgs& sg, int blk,
     if (c >= n_live) continue;
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp)
-      sg.part[(3 * static_cast<int64_t>(chunk) + comp) * sg.n + c] =
+      sg.part[(3 * static_cast<int64_t>(chunk) + comp) * sg.pstride + c] =
           S[k][comp];
   }
+}
+
+// One target of a source-split sum: S = the ks chunk partials at part[0],
+// part[pstride], ... added in chunk order, and o = x cross S (x at p, its
+// components cs apart; o's ocs apart).  Every split reduction goes through
+// this function, so equal partials give equal bits.
+__device__ __forceinline__ void split_reduce_one(const double* part,
+                                                 int64_t pstride, int ks,
+                                                 const double* p, int64_t cs,
+                                                 double* o, int64_t ocs) {
+  double S[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < ks; ++k)
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+      S[comp] += part[(3 * static_cast<int64_t>(k) + comp) * pstride];
+  const double x0 = p[0], x1 = p[cs], x2 = p[2 * cs];
+  o[0] = x1 * S[2] - x2 * S[1];
+  o[ocs] = x2 * S[0] - x0 * S[2];
+  o[2 * ocs] = x0 * S[1] - x1 * S[0];
 }
 
 // The packed s y of source j (SY), or any source record the caller passes

@@ -186,17 +186,8 @@ __global__ __launch_bounds__(256) void bve_split_reduce_kernel(SegArgs s0,
   if (c >= n_live) return;
   const int64_t i = sg.index ? sg.index[c] : sg.slot_base + c;
   if (sg.skip && sg.skip[i]) return;
-  double S[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < ks; ++k)
-#pragma unroll
-    for (int comp = 0; comp < 3; ++comp)
-      S[comp] += sg.part[(3 * static_cast<int64_t>(k) + comp) * sg.n + c];
-  const double* p = sg.x + i * sg.es;
-  const double x[1][3] = {{p[0], p[sg.cs], p[2 * sg.cs]}};
-  const double SS[1][3] = {{S[0], S[1], S[2]}};
-  const bool live[1] = {true};
-  const int64_t tgt[1] = {i};
-  store_cross<1>(sg, x, SS, live, tgt);
+  split_reduce_one(sg.part + c, sg.pstride, ks, sg.x + i * sg.es, sg.cs,
+                   sg.out + i * sg.oes, sg.ocs);
 }
 
 template <int BLOCK, int TPL, int TILE, bool EPS, bool PAIR>
@@ -531,6 +522,7 @@ SegArgs to_args(const TargetSeg& t, int per_block) {
   a.oes = t.out.es;
   a.ocs = t.out.cs;
   a.out_scalar = t.out_scalar;
+  a.pstride = t.n;
   return a;
 }
 
