@@ -527,7 +527,7 @@ int lpm_bve_rk4_step(lpm_ctx* ctx, const lpm_bve_fields* f, double dt,
       const int64_t nt = f->n_verts + f->n_faces;
       const size_t part =
           bve_split_bytes(precision, bve_velocity_variant(ctx, precision, nt), nt);
-      if (part) ctx->split_part.ensure(part);
+      if (part) split_workspace(ctx, part, s);  // nullptr: runs unsplit
       return 0;
     };
     run_step(ctx, *f, dt, Omega, precision, flags, s, prepare,

@@ -739,6 +739,20 @@ size_t bve_split_bytes(int precision, int variant, int64_t n_targets) {
          static_cast<size_t>(std::max<int64_t>(n_targets, 1));
 }
 
+double* split_workspace(lpm_ctx* ctx, size_t need, hipStream_t s) {
+  if (ctx->split_part.bytes >= need) return ctx->split_part.as<double>();
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  LPM_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return nullptr;
+  try {
+    return static_cast<double*>(ctx->split_part.ensure(need));
+  } catch (const Error& e) {
+    if (e.code != LPM_ERR_NOMEM) throw;
+    (void)hipGetLastError();  // the failed hipMalloc's error is handled
+    return nullptr;
+  }
+}
+
 namespace {
 
 void launch_split_reduce(const SegArgs& a0, const SegArgs& a1, int ks,
@@ -773,16 +787,12 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
   int twin = -1;
   if (split_of(precision, variant, &twin) > 1) {
     // the partial sums live in a context workspace: a step's prepare sizes
-    // it before capture; a call that may not allocate it (no permission, or
-    // capture with it too small) runs the unsplit twin
-    const size_t need = bve_split_bytes(precision, variant, t0.n + t1.n);
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    LPM_HIP_CHECK(hipStreamIsCapturing(s, &cs));
-    if (allow_split && (ctx->split_part.bytes >= need ||
-                        cs == hipStreamCaptureStatusNone))
-      part = static_cast<double*>(ctx->split_part.ensure(need));
-    else
-      variant = twin;
+    // it before capture; a call that may not allocate it (no permission,
+    // capture with it too small, or no memory for it) runs the unsplit twin
+    if (allow_split)
+      part = split_workspace(
+          ctx, bve_split_bytes(precision, variant, t0.n + t1.n), s);
+    if (!part) variant = twin;
   }
   if (precision == LPM_FP64) {
     const F64Variant& v = kF64Variants[variant];

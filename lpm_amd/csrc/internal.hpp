@@ -308,9 +308,13 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
                          hipStream_t s, int variant = -1, bool timed = true,
                          bool allow_split = false);
-// Bytes of partial sums a source-split FP64 variant needs for n_targets
-// (0 for every other variant).
+// Bytes of partial sums a source-split variant needs for n_targets (0 for
+// every other variant).
 size_t bve_split_bytes(int precision, int variant, int64_t n_targets);
+// The context's split workspace grown to `need` bytes, or nullptr when it
+// cannot be: s is capturing and it is smaller, or the device is out of
+// memory (the caller then runs an unsplit variant).
+double* split_workspace(lpm_ctx* ctx, size_t need, hipStream_t s);
 void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s);
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,

@@ -456,6 +456,50 @@ def test_graph_recaptured_after_variant_change():
         c.close()
 
 
+def test_split_variant_under_a_callers_capture_runs_its_twin():
+    """A source-split variant cannot grow its workspace inside a capture the
+    library did not start: in a fresh context, a split init_velocity
+    captured by the caller runs the unsplit twin (bit-equal to it on every
+    replay), and the next eager call allocates the workspace and splits
+    (same sums to 1e-13, another summation order)."""
+    from lpm_amd.device import Context
+    c = Context(0)
+    kid = _abi.LPM_KERNEL_BVE_VEL_F64
+    names = c.variants(kid)
+    v = names.index("block256_tpl4_nr1_quad_fused_sy_u2_tile512_split32")
+    twin = names.index("block256_tpl4_nr1_quad_fused_sy_u2_tile512")
+    try:
+        # level 5: 40 source tiles, so some of the 32 chunks hold two and
+        # the split sum's association differs from the unsplit one
+        a, b = _sphere(c, 0, 5), _sphere(c, 0, 5)
+        c.set_variant(kid, twin)
+        a.init_velocity()
+        b.init_velocity()  # sizes every workspace but the split one
+        c.set_variant(kid, v)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(g, stream=side):
+            b.init_velocity()
+        for _ in range(3):
+            b.velocity_verts.zero_()
+            b.velocity_faces.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            _ = (a.vert_x * 2).sum().item()  # eager work between replays
+            assert torch.equal(a.velocity_verts, b.velocity_verts)
+            assert _same_bits(a.velocity_faces, b.velocity_faces)
+        del g
+        b.init_velocity()
+        torch.cuda.synchronize()
+        assert not torch.equal(a.velocity_verts, b.velocity_verts)
+        assert rel_l2(b.velocity_verts.cpu().numpy(),
+                      a.velocity_verts.cpu().numpy()) < 1e-13
+    finally:
+        c.close()
+
+
 def _same_bits(p, q):
     """Equal values, with NaN where the other has NaN."""
     return bool(((p == q) | (p.isnan() & q.isnan())).all())
