@@ -314,19 +314,21 @@ __device__ __forceinline__ void store_partial(const SegArgs& sg, int blk,
   }
 }
 
-// One target of a source-split sum: S = the ks chunk partials at part[0],
-// part[pstride], ... added in chunk order, and o = x cross S (x at p, its
-// components cs apart; o's ocs apart).  Every split reduction goes through
-// this function, so equal partials give equal bits.
+// One target of a source-split sum: S = the ks chunk partials (chunk k's
+// component comp at part[(nper k + comp) pstride], nper values per chunk)
+// added in chunk order, and o = x cross S (x at p, its components cs apart;
+// o's ocs apart).  Every split reduction goes through this function, so
+// equal partials give equal bits.
 __device__ __forceinline__ void split_reduce_one(const double* part,
                                                  int64_t pstride, int ks,
                                                  const double* p, int64_t cs,
-                                                 double* o, int64_t ocs) {
+                                                 double* o, int64_t ocs,
+                                                 int nper = 3) {
   double S[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < ks; ++k)
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp)
-      S[comp] += part[(3 * static_cast<int64_t>(k) + comp) * pstride];
+      S[comp] += part[(static_cast<int64_t>(nper) * k + comp) * pstride];
   const double x0 = p[0], x1 = p[cs], x2 = p[2 * cs];
   o[0] = x1 * S[2] - x2 * S[1];
   o[ocs] = x2 * S[0] - x0 * S[2];

@@ -79,16 +79,55 @@ __device__ __forceinline__ void ic2d_tile_sum(const double4* tile,
   psi += P;
 }
 
+// Source chunk `chunk` of ks equal runs of packing tiles: [*jbeg, *jend).
+__device__ __forceinline__ void ic2d_chunk(int nsrc, int ks, int chunk,
+                                           int* jbeg, int* jend) {
+  const int ntile = (nsrc + kPackTile - 1) / kPackTile;
+  *jbeg = (ntile * chunk) / ks * kPackTile;
+  *jend = (ntile * (chunk + 1)) / ks * kPackTile;
+}
+
+// Stores S and psi of slot blk * BLOCK + threadIdx.x: to the chunk's
+// partials when the launch is source-split (sg.part), else u = x cross S
+// and psi at a live target.
+template <int BLOCK>
+__device__ __forceinline__ void ic2d_store(const SegArgs& sg, int blk,
+                                           int chunk, const double (*x)[3],
+                                           const double (*S)[3], double psi,
+                                           const bool* live,
+                                           const int64_t* tgt) {
+  if (sg.part) {
+    const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+    const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
+    if (c >= n_live) return;
+    double* q = sg.part + c;
+    const int64_t k4 = 4 * static_cast<int64_t>(chunk);
+    q[(k4 + 0) * sg.pstride] = S[0][0];
+    q[(k4 + 1) * sg.pstride] = S[0][1];
+    q[(k4 + 2) * sg.pstride] = S[0][2];
+    q[(k4 + 3) * sg.pstride] = psi;
+    return;
+  }
+  if (sg.out) store_cross<1>(sg, x, S, live, tgt);
+  if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
+}
+
+// Block b * ks + k sums source chunk k for target block b (ks = 1: all
+// sources, results stored directly).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
-    SegArgs s0, SegArgs s1, double eps2, int skip_self) {
+    SegArgs s0, SegArgs s1, double eps2, int skip_self, int ks) {
   __shared__ double4 tile[kPackTile];
   __shared__ double2 ltab[kLogTable];
-  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const int b = static_cast<int>(blockIdx.x) / ks;
+  const int chunk = static_cast<int>(blockIdx.x) % ks;
+  const bool first = b < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
-  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  const int blk = first ? b : b - s0.nblocks;
   const int nsrc = *count;
+  int jbeg, jend;
+  ic2d_chunk(nsrc, ks, chunk, &jbeg, &jend);
   double x[1][3];
   int self[1];
   bool live[1];
@@ -98,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
   double S[1][3] = {{0.0, 0.0, 0.0}};
   double psi = 0.0;
   fill_log_table(ltab);
-  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+  for (int j0 = jbeg; j0 < jend; j0 += kPackTile) {
     __syncthreads();
     for (int t = threadIdx.x; t < kPackTile; t += BLOCK) {
       const int j = j0 + t;
@@ -111,20 +150,23 @@ __global__ __launch_bounds__(BLOCK) void ic2d_f64_kernel(
     else
       ic2d_tile_sum<false>(tile, x[0], 1.0 + eps2, so, ltab, S[0], psi);
   }
-  if (sg.out) store_cross<1>(sg, x, S, live, tgt);
-  if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
+  ic2d_store<BLOCK>(sg, blk, chunk, x, S, psi, live, tgt);
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void ic2d_mixed_kernel(
     const float4* __restrict__ src, const double4* __restrict__ centers,
     const int32_t* __restrict__ count, SegArgs s0, SegArgs s1, float e2x2,
-    float e2, int skip_self) {
+    float e2, int skip_self, int ks) {
   __shared__ float4 tile[kPackTile];
-  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const int b = static_cast<int>(blockIdx.x) / ks;
+  const int chunk = static_cast<int>(blockIdx.x) % ks;
+  const bool first = b < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
-  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  const int blk = first ? b : b - s0.nblocks;
   const int nsrc = *count;
+  int jbeg, jend;
+  ic2d_chunk(nsrc, ks, chunk, &jbeg, &jend);
   double x[1][3];
   int self[1];
   bool live[1];
@@ -134,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) void ic2d_mixed_kernel(
   double S[1][3] = {{0.0, 0.0, 0.0}};
   double psi = 0.0;
   constexpr float kLn2 = 0.693147180559945309f;
-  for (int j0 = 0; j0 < nsrc; j0 += kPackTile) {
+  for (int j0 = jbeg; j0 < jend; j0 += kPackTile) {
     __syncthreads();
     for (int t = threadIdx.x; t < kPackTile; t += BLOCK) {
       const int j = j0 + t;
@@ -170,8 +212,32 @@ __global__ __launch_bounds__(BLOCK) void ic2d_mixed_kernel(
     S[0][2] += static_cast<double>(T2);
     psi += static_cast<double>(P) * (0.5 * static_cast<double>(kLn2));
   }
-  if (sg.out) store_cross<1>(sg, x, S, live, tgt);
-  if (sg.out_scalar && live[0]) sg.out_scalar[tgt[0]] = psi;
+  ic2d_store<BLOCK>(sg, blk, chunk, x, S, psi, live, tgt);
+}
+
+// u = x cross (sum of the ks chunk partials of S, in chunk order) and psi
+// = the sum of the chunk partials of psi, at every live slot.
+__global__ __launch_bounds__(256) void ic2d_split_reduce_kernel(SegArgs s0,
+                                                                SegArgs s1,
+                                                                int ks) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const bool first = g < s0.n;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int64_t c = first ? g : g - s0.n;
+  const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+  if (c >= n_live) return;
+  const int64_t i = sg.index ? sg.index[c] : sg.slot_base + c;
+  if (sg.skip && sg.skip[i]) return;
+  const double* q = sg.part + c;
+  if (sg.out)
+    split_reduce_one(q, sg.pstride, ks, sg.x + i * sg.es, sg.cs,
+                     sg.out + i * sg.oes, sg.ocs, 4);
+  if (sg.out_scalar) {
+    double psi = 0.0;
+    for (int k = 0; k < ks; ++k)
+      psi += q[(4 * static_cast<int64_t>(k) + 3) * sg.pstride];
+    sg.out_scalar[i] = psi;
+  }
 }
 
 // Stage storage of one particle kind (SoA, leading dimension n): x1 (3n),
@@ -253,6 +319,36 @@ __global__ __launch_bounds__(256) void ic2d_update_kernel(Ic2dArgs A) {
 
 inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
+constexpr int kIc2dBlock = 256;
+
+// Source chunks per target block: with fewer target blocks than eight
+// rounds of resident blocks the launch splits each block's sources (at
+// least 8 packing tiles per chunk) so that every CU holds enough waves to
+// hide the log's latency to the end of the launch (level 6: 2, 4 and 8
+// rounds measured 823, 898 and 913 Gpairs/s in FP64, 746 unsplit).
+int ic2d_chunks(lpm_ctx* ctx, int precision, int64_t blocks,
+                int64_t nsrc_upper) {
+  const void* fn =
+      precision == LPM_MIXED
+          ? reinterpret_cast<const void*>(&ic2d_mixed_kernel<kIc2dBlock>)
+          : reinterpret_cast<const void*>(&ic2d_f64_kernel<kIc2dBlock>);
+  int per_cu = 0;
+  LPM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
+                                                             kIc2dBlock, 0));
+  const int64_t want = 8 * static_cast<int64_t>(std::max(per_cu, 1)) *
+                       std::max(ctx->num_cus, 1);
+  const int64_t ntile = (nsrc_upper + kPackTile - 1) / kPackTile;
+  if (blocks <= 0 || blocks >= want || ntile < 16) return 1;
+  const int64_t ks = std::min((want + blocks - 1) / blocks, ntile / 8);
+  return static_cast<int>(std::max<int64_t>(ks, 1));
+}
+
+size_t ic2d_split_bytes(int ks, int64_t n_targets) {
+  return ks > 1 ? sizeof(double) * 4 * static_cast<size_t>(ks) *
+                      static_cast<size_t>(std::max<int64_t>(n_targets, 1))
+                : 0;
+}
+
 void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
                  int nseg, double eps, bool collocated_skip, int precision,
                  hipStream_t s) {
@@ -260,7 +356,7 @@ void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
   const TargetSeg& t0 = segs[0];
   const TargetSeg& t1 = nseg > 1 ? segs[1] : empty;
   if (t0.n + t1.n == 0) return;
-  constexpr int kBlock = 256;
+  constexpr int kBlock = kIc2dBlock;
   auto to_args = [](const TargetSeg& t) {
     SegArgs a{};
     a.x = t.x.base;
@@ -277,24 +373,44 @@ void launch_ic2d(lpm_ctx* ctx, const PackedSources& src, TargetSeg* segs,
     a.oes = t.out.es;
     a.ocs = t.out.cs;
     a.out_scalar = t.out_scalar;
+    a.pstride = t.n;
     return a;
   };
   SegArgs a0 = to_args(t0), a1 = to_args(t1);
+  const int blocks = a0.nblocks + a1.nblocks;
+  int ks = ic2d_chunks(ctx, precision, blocks, src.capacity);
+  if (ks > 1) {
+    // no workspace (a caller's capture, no memory): one chunk
+    double* part = split_workspace(ctx, ic2d_split_bytes(ks, t0.n + t1.n), s);
+    if (part) {
+      a0.part = part;
+      a1.part = part + 4 * static_cast<int64_t>(ks) * t0.n;
+    } else {
+      ks = 1;
+    }
+  }
   const double pairs = static_cast<double>(t0.n + t1.n) * src.capacity;
   TimedLaunch tl(ctx, KID_IC2D, s, pairs);
+  const dim3 grid(static_cast<unsigned>(blocks) * ks);
   if (precision == LPM_MIXED) {
-    hipLaunchKernelGGL((ic2d_mixed_kernel<kBlock>),
-                       dim3(a0.nblocks + a1.nblocks), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((ic2d_mixed_kernel<kBlock>), grid, dim3(kBlock), 0, s,
                        static_cast<const float4*>(src.data), src.centers,
                        src.count, a0, a1, static_cast<float>(2.0 * eps * eps),
-                       static_cast<float>(eps * eps), collocated_skip ? 1 : 0);
+                       static_cast<float>(eps * eps), collocated_skip ? 1 : 0,
+                       ks);
   } else {
-    hipLaunchKernelGGL((ic2d_f64_kernel<kBlock>),
-                       dim3(a0.nblocks + a1.nblocks), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((ic2d_f64_kernel<kBlock>), grid, dim3(kBlock), 0, s,
                        static_cast<const double4*>(src.data), src.count, a0,
-                       a1, eps * eps, collocated_skip ? 1 : 0);
+                       a1, eps * eps, collocated_skip ? 1 : 0, ks);
   }
   LPM_HIP_CHECK(hipGetLastError());
+  if (ks > 1) {
+    const int64_t nt = t0.n + t1.n;
+    hipLaunchKernelGGL(ic2d_split_reduce_kernel,
+                       dim3(static_cast<unsigned>((nt + 255) / 256)),
+                       dim3(256), 0, s, a0, a1, ks);
+    LPM_HIP_CHECK(hipGetLastError());
+  }
 }
 
 // Sums of the fields at positions (vx, fx) with face vorticity fzeta.
@@ -511,6 +627,12 @@ int lpm_ic2d_rk2_step(lpm_ctx* ctx, const lpm_ic2d_fields* f, double dt,
           const RankMap map = build_rank(ctx, f->face_mask, f->n_faces, s);
           pack_sources(ctx, CrdIn{f->face_x, f->face_x_es, f->face_x_cs},
                        f->face_zeta, f->face_area, map, precision, s);
+          // the split workspace of the two evaluations, before any capture
+          const int ks = ic2d_chunks(
+              ctx, precision, nblk(f->n_verts) + nblk(f->n_faces), f->n_faces);
+          if (ks > 1)
+            split_workspace(ctx, ic2d_split_bytes(ks, f->n_verts + f->n_faces),
+                            s);
           return ic2d_key(*f, dt, Omega, eps, precision, flags, s);
         },
         [&]() {

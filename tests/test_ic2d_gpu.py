@@ -31,10 +31,14 @@ def dev(a, dtype=torch.float64):
     return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
 
 
-@pytest.mark.parametrize("seed", [0, 1])
+# level 3 runs each target block over all sources; at level 5 (80 or 24
+# source tiles of 256 behind 148 or 57 target blocks) the launch splits
+# every block's sources into chunks and adds the chunk partials in a second
+# kernel
+@pytest.mark.parametrize("seed,depth", [(0, 3), (1, 3), (0, 5), (1, 5)])
 @pytest.mark.parametrize("eps", [0.0, 0.05])
-def test_sums_match_oracle(ctx, ora, seed, eps):
-    m = mesh(seed, 3)
+def test_sums_match_oracle(ctx, ora, seed, depth, eps):
+    m = mesh(seed, depth)
     z = G.rh54_vorticity(m.face_x, 0.1, -0.1)
     mask = dev(m.face_mask, torch.uint8)
     u, psi = ctx.ic2d_sums(dev(m.vert_x), dev(m.face_x), dev(z),
@@ -89,10 +93,11 @@ def test_rk2_matches_oracle(ctx, ora, eps):
         assert rel_l2(got[k], st[k]) < 1e-11, k
 
 
-def test_rk2_graph_replays_match_eager(ctx):
+@pytest.mark.parametrize("depth", [3, 5])
+def test_rk2_graph_replays_match_eager(ctx, depth):
     from lpm_amd.incompressible2d import Incompressible2DRK2
-    a = _model(ctx, 0, 3, 0.0, 2 * np.pi)
-    b = _model(ctx, 0, 3, 0.0, 2 * np.pi)
+    a = _model(ctx, 0, depth, 0.0, 2 * np.pi)
+    b = _model(ctx, 0, depth, 0.0, 2 * np.pi)
     skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
     a.init_direct_sums(skip)
     b.init_direct_sums(skip)
