@@ -286,16 +286,23 @@ __device__ __forceinline__ double psi_tile_sum(const double4* tile, double x0,
   return p;
 }
 
+// Block b * ks + k sums source chunk k (of ks equal runs of tiles) for
+// target block b: into sg.part[k pstride + c] when split, else psi itself.
 template <int BLOCK, int TILE>
 __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
     const double4* __restrict__ src, const int32_t* __restrict__ count,
-    SegArgs s0, SegArgs s1) {
+    SegArgs s0, SegArgs s1, int ks) {
   __shared__ double4 tile[TILE];
   __shared__ double2 ltab[kLogTable];
-  const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
+  const int b = static_cast<int>(blockIdx.x) / ks;
+  const int chunk = static_cast<int>(blockIdx.x) % ks;
+  const bool first = b < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
-  const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
+  const int blk = first ? b : b - s0.nblocks;
   const int nsrc = *count;
+  const int ntile = (nsrc + TILE - 1) / TILE;
+  const int jbeg = (ntile * chunk) / ks * TILE;
+  const int jend = (ntile * (chunk + 1)) / ks * TILE;
   const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
   if (static_cast<int64_t>(blk) * BLOCK >= n_live) return;
   const int64_t c = static_cast<int64_t>(blk) * BLOCK + threadIdx.x;
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
   }
   double psi = 0.0;
   fill_log_table(ltab);
-  for (int j0 = 0; j0 < nsrc; j0 += TILE) {
+  for (int j0 = jbeg; j0 < jend; j0 += TILE) {
     __syncthreads();
     for (int t = threadIdx.x; t < TILE; t += BLOCK) {
       const int j = j0 + t;
@@ -329,7 +336,28 @@ __global__ __launch_bounds__(BLOCK) void bve_psi_f64_kernel(
     else
       psi += psi_tile_sum<false, TILE>(tile, x0, x1, x2, so, ltab);
   }
-  if (live) sg.out_scalar[i] = psi;
+  if (sg.part) {
+    if (in) sg.part[chunk * sg.pstride + c] = psi;
+  } else if (live) {
+    sg.out_scalar[i] = psi;
+  }
+}
+
+// psi = the ks chunk partials added in chunk order, at every live slot.
+__global__ __launch_bounds__(256) void bve_psi_reduce_kernel(SegArgs s0,
+                                                             SegArgs s1,
+                                                             int ks) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const bool first = g < s0.n;
+  const SegArgs& sg = pick(s0, s1, first);
+  const int64_t c = first ? g : g - s0.n;
+  const int64_t n_live = sg.d_n ? *sg.d_n : sg.n;
+  if (c >= n_live) return;
+  const int64_t i = sg.index ? sg.index[c] : sg.slot_base + c;
+  if (sg.skip && sg.skip[i]) return;
+  double psi = 0.0;
+  for (int k = 0; k < ks; ++k) psi += sg.part[k * sg.pstride + c];
+  sg.out_scalar[i] = psi;
 }
 
 __global__ __launch_bounds__(256) void finite_kernel(
@@ -852,19 +880,55 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
   }
 }
 
+int split_chunks(lpm_ctx* ctx, const void* kernel, int block, int64_t blocks,
+                 int64_t nsrc_upper, int tile) {
+  int per_cu = 0;
+  LPM_HIP_CHECK(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+  const int64_t want = 8 * static_cast<int64_t>(std::max(per_cu, 1)) *
+                       std::max(ctx->num_cus, 1);
+  const int64_t ntile = (nsrc_upper + tile - 1) / tile;
+  if (blocks <= 0 || blocks >= want || ntile < 16) return 1;
+  const int64_t ks = std::min((want + blocks - 1) / blocks, ntile / 8);
+  return static_cast<int>(std::max<int64_t>(ks, 1));
+}
+
 void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s) {
   TargetSeg empty;
   const TargetSeg& t0 = segs[0];
   const TargetSeg& t1 = nseg > 1 ? segs[1] : empty;
   if (t0.n + t1.n == 0) return;
-  constexpr int kBlock = 256;
+  constexpr int kBlock = 256, kTile = 256;
   SegArgs a0 = to_args(t0, kBlock), a1 = to_args(t1, kBlock);
+  const int blocks = a0.nblocks + a1.nblocks;
+  int ks = split_chunks(
+      ctx, reinterpret_cast<const void*>(&bve_psi_f64_kernel<kBlock, kTile>),
+      kBlock, blocks, src.capacity, kTile);
+  if (ks > 1) {
+    // no workspace (a caller's capture, no memory): one chunk
+    double* part = split_workspace(
+        ctx, sizeof(double) * static_cast<size_t>(ks) * (t0.n + t1.n), s);
+    if (part) {
+      a0.part = part;
+      a1.part = part + static_cast<int64_t>(ks) * t0.n;
+    } else {
+      ks = 1;
+    }
+  }
   TimedLaunch tl(ctx, KID_BVE_PSI, s, count_pairs(segs, nseg, src.capacity));
-  hipLaunchKernelGGL((bve_psi_f64_kernel<kBlock, 256>),
-                     dim3(a0.nblocks + a1.nblocks), dim3(kBlock), 0, s,
-                     static_cast<const double4*>(src.data), src.count, a0, a1);
+  hipLaunchKernelGGL((bve_psi_f64_kernel<kBlock, kTile>),
+                     dim3(static_cast<unsigned>(blocks) * ks), dim3(kBlock), 0,
+                     s, static_cast<const double4*>(src.data), src.count, a0,
+                     a1, ks);
   LPM_HIP_CHECK(hipGetLastError());
+  if (ks > 1) {
+    const int64_t nt = t0.n + t1.n;
+    hipLaunchKernelGGL(bve_psi_reduce_kernel,
+                       dim3(static_cast<unsigned>((nt + 255) / 256)),
+                       dim3(256), 0, s, a0, a1, ks);
+    LPM_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,

@@ -321,26 +321,18 @@ inline int nblk(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
 constexpr int kIc2dBlock = 256;
 
-// Source chunks per target block: with fewer target blocks than eight
-// rounds of resident blocks the launch splits each block's sources (at
-// least 8 packing tiles per chunk) so that every CU holds enough waves to
-// hide the log's latency to the end of the launch (level 6: 2, 4 and 8
-// rounds measured 823, 898 and 913 Gpairs/s in FP64, 746 unsplit).
+// Source chunks per target block (split_chunks): with the level-6 sums'
+// 588 target blocks far below the blocks the chip holds at once, each
+// block's sources are split so that every CU holds enough waves to hide
+// the log's latency to the end of the launch (level 6: 2, 4 and 8 rounds of
+// resident blocks measured 823, 898 and 913 Gpairs/s in FP64, 746 unsplit).
 int ic2d_chunks(lpm_ctx* ctx, int precision, int64_t blocks,
                 int64_t nsrc_upper) {
   const void* fn =
       precision == LPM_MIXED
           ? reinterpret_cast<const void*>(&ic2d_mixed_kernel<kIc2dBlock>)
           : reinterpret_cast<const void*>(&ic2d_f64_kernel<kIc2dBlock>);
-  int per_cu = 0;
-  LPM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
-                                                             kIc2dBlock, 0));
-  const int64_t want = 8 * static_cast<int64_t>(std::max(per_cu, 1)) *
-                       std::max(ctx->num_cus, 1);
-  const int64_t ntile = (nsrc_upper + kPackTile - 1) / kPackTile;
-  if (blocks <= 0 || blocks >= want || ntile < 16) return 1;
-  const int64_t ks = std::min((want + blocks - 1) / blocks, ntile / 8);
-  return static_cast<int>(std::max<int64_t>(ks, 1));
+  return split_chunks(ctx, fn, kIc2dBlock, blocks, nsrc_upper, kPackTile);
 }
 
 size_t ic2d_split_bytes(int ks, int64_t n_targets) {

@@ -315,6 +315,12 @@ size_t bve_split_bytes(int precision, int variant, int64_t n_targets);
 // cannot be: s is capturing and it is smaller, or the device is out of
 // memory (the caller then runs an unsplit variant).
 double* split_workspace(lpm_ctx* ctx, size_t need, hipStream_t s);
+// Source chunks per target block for a launch of `blocks` target blocks of
+// `kernel` over nsrc_upper sources in tiles of `tile`: enough that the
+// items fill eight rounds of the blocks the device holds at once, with at
+// least 8 tiles per chunk (1: no split).
+int split_chunks(lpm_ctx* ctx, const void* kernel, int block, int64_t blocks,
+                 int64_t nsrc_upper, int tile);
 void launch_bve_stream_fn(lpm_ctx* ctx, const PackedSources& src,
                           TargetSeg* segs, int nseg, hipStream_t s);
 void check_finite(lpm_ctx* ctx, const double* base, int64_t n, int64_t es,

@@ -114,8 +114,11 @@ def test_velocity_mixed_matches_fp64(ctx):
         assert e < TOL_MIXED, e
 
 
-def test_stream_function_matches_oracle(ctx, ora):
-    m, z = _inputs(0, 3)
+# level 5: the vertex launch has 41 target blocks, so every block's sources
+# are split into chunks whose partials a second kernel adds
+@pytest.mark.parametrize("seed,depth", [(0, 3), (0, 5), (1, 5)])
+def test_stream_function_matches_oracle(ctx, ora, seed, depth):
+    m, z = _inputs(seed, depth)
     psi = ctx.bve_stream_fn(dev(m.vert_x), dev(m.face_x), dev(z),
                             dev(m.face_area), dev(m.face_mask, torch.uint8))
     ref = ora.bve_stream_fn(aos(m.vert_x), aos(m.face_x), z, m.face_area,
