@@ -27,6 +27,15 @@
  *    it after the previous stream's work (an event wait, no host sync).  A
  *    call on a stream that is being captured by the caller is not ordered
  *    this way: order such streams yourself.
+ *  - Sums whose target blocks would leave the device underfilled split each
+ *    block's sources into chunks and add the chunk partials in a second
+ *    kernel (the stream function and Incompressible2D sums always when
+ *    they have fewer target blocks than eight rounds of the blocks the
+ *    device holds at once; BVE velocities through the *_splitK variants).
+ *    The partials live in a context workspace grown on demand; a call that
+ *    cannot grow it (on a stream the caller is capturing, or out of device
+ *    memory) runs unsplit.  The chunk count follows from the problem size
+ *    and the device, so results are reproducible on one device.
  */
 #ifndef LPM_MI355X_H
 #define LPM_MI355X_H
