@@ -807,6 +807,8 @@ void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                   "target count out of range");
   if (t0.n + t1.n == 0) return;
   const double pairs = count_pairs(segs, nseg, src.capacity);
+  LPM_ARG_CHECK(precision == LPM_FP64 || precision == LPM_MIXED,
+                "unknown precision");
   if (variant < 0) variant = bve_velocity_variant(ctx, precision, t0.n + t1.n);
   LPM_ARG_CHECK(variant >= 0 && variant < (precision == LPM_FP64 ? kNumF64
                                                                   : kNumMixed),

@@ -155,7 +155,7 @@ struct lpm_ctx {
   lpm_amd::DevBuf block_counts; // int32 per scan block (+1 total)
   lpm_amd::DevBuf packed64;     // double4 per source
   lpm_amd::DevBuf packed64sy;   // double4 {s y, 0} per source (FP64)
-  lpm_amd::DevBuf split_part;   // partial sums of source-split FP64 sums
+  lpm_amd::DevBuf split_part;   // chunk partials of source-split sums
   lpm_amd::DevBuf packed32;     // float4 per source (mixed)
   lpm_amd::DevBuf centers;      // double4 per tile (mixed)
   lpm_amd::DevBuf mf_y, mf_q, mf_s, mf_t, mf_c;  // matrix-core layout (mixed)
@@ -302,8 +302,8 @@ unsigned bve_variant_layouts(int precision, int variant);
 TargetSeg face_targets(CrdIn x, const RankMap& map, const uint8_t* mask,
                        bool skip_masked, CrdOut out, double* out_scalar);
 // variant < 0: bve_velocity_variant's choice.  timed = false skips the
-// timing events.  allow_split = false runs a source-split FP64 variant as
-// its unsplit twin (same shape, one summation level less, no workspace).
+// timing events.  allow_split = false runs a source-split variant as its
+// unsplit twin (same shape, one summation level less, no workspace).
 void launch_bve_velocity(lpm_ctx* ctx, const PackedSources& src,
                          TargetSeg* segs, int nseg, double eps2, int precision,
                          hipStream_t s, int variant = -1, bool timed = true,
