@@ -956,8 +956,13 @@ __device__ __forceinline__ void mfma16_targets(double e0, double e1, double e2,
 
 // One 32-source chunk: the distance products of both target blocks, their
 // reciprocals and the weight products into D2.  la / lt address the chunk's
-// sources (32 sources: 64 uint4 of A, 32 float4 of T).
-template <int NACC>
+// sources (32 sources: 64 uint4 of A, 32 float4 of T).  RS: distances per
+// reciprocal: 2 gives 1/Da = Db / (Da Db), 4 shares one reciprocal of
+// (Da Db)(Dc Dd) the same way (multiplies for quarter-rate reciprocals).
+// Far-tile distances are scaled by at most 1/kMfma16MinScale^2 and lie in
+// about [0.2, 4e4] even on very fine meshes, so a product of four stays far
+// inside the FP32 range.
+template <int NACC, int RS = 1>
 __device__ __forceinline__ void mfma16_chunk(const uint4* la, const float4* lt,
                                              int col, int h, int c4,
                                              const mh8* B, mf4 (*D2)[NACC]) {
@@ -969,10 +974,33 @@ __device__ __forceinline__ void mfma16_chunk(const uint4* la, const float4* lt,
 #pragma unroll
   for (int tb = 0; tb < 2; ++tb) {
     const mf16 D = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B[tb], mf16{}, 0, 0, 0);
+    float w[16];
+    if constexpr (RS == 4) {
+#pragma unroll
+      for (int v = 0; v < 16; v += 4) {
+        const float pab = D[v] * D[v + 1], pcd = D[v + 2] * D[v + 3];
+        const float r = __builtin_amdgcn_rcpf(pab * pcd);
+        const float rab = r * pcd, rcd = r * pab;
+        w[v] = D[v + 1] * rab;
+        w[v + 1] = D[v] * rab;
+        w[v + 2] = D[v + 3] * rcd;
+        w[v + 3] = D[v + 2] * rcd;
+      }
+    } else if constexpr (RS == 2) {
+#pragma unroll
+      for (int v = 0; v < 16; v += 2) {
+        const float r = __builtin_amdgcn_rcpf(D[v] * D[v + 1]);
+        w[v] = D[v + 1] * r;
+        w[v + 1] = D[v] * r;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) w[v] = __builtin_amdgcn_rcpf(D[v]);
+    }
 #pragma unroll
     for (int v = 0; v < 16; ++v)
       D2[tb][v % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(
-          tv[v], __builtin_amdgcn_rcpf(D[v]), D2[tb][v % NACC], 0, 0, 0);
+          tv[v], w[v], D2[tb][v % NACC], 0, 0, 0);
   }
 }
 
@@ -998,7 +1026,7 @@ __device__ __forceinline__ void mfma16_reduce(mf4 (*D2)[NACC], int h, float* o) 
 // NACC: independent weight-product accumulators per target block (1 or 2;
 // two shorten the dependent MFMA chain).  KS > 1: only source chunk `chunk`
 // of KS equal runs of packing tiles, sums to sg.part (as f64_block).
-template <bool EPS, int NACC = 1, int KS = 1>
+template <bool EPS, int NACC = 1, int KS = 1, int RS = 1>
 __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
                                              const SegArgs& sg, int blk,
                                              float eps2x2, float4* ly,
@@ -1052,7 +1080,7 @@ __device__ __forceinline__ void mfma16_block(const Mfma16Src& src, int nsrc,
       for (int q = 0; q < NACC; ++q) D2[tb][q] = mf4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int ch = 0; ch < kPackTile / 32; ++ch)
-      mfma16_chunk<NACC>(la + 64 * ch, lt + 32 * ch, col, h, c4, B, D2);
+      mfma16_chunk<NACC, RS>(la + 64 * ch, lt + 32 * ch, col, h, c4, B, D2);
     float o[4];
     mfma16_reduce<NACC>(D2, h, o);
     const double W = static_cast<double>(o[3]);

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void ens_mfma_kernel(MfmaSrc src, EnsArgs A) {
   }
 }
 
-template <int MINW, int NACC = 1>
+template <int MINW, int NACC = 1, int RS = 1>
 __global__ __launch_bounds__(256, MINW) void ens_mfma16_kernel(Mfma16Src src,
                                                                EnsArgs A) {
   __shared__ float4 ly[kPackTile];
@@ -183,7 +183,8 @@ __global__ __launch_bounds__(256, MINW) void ens_mfma16_kernel(Mfma16Src src,
     Mfma16Src ms{src.y + M.P, src.s + M.P, src.a + 2 * static_cast<int64_t>(M.P),
                  src.t + static_cast<int64_t>(t0) * 1024, src.c + t0,
                  src.is + t0};
-    mfma16_block<false, NACC>(ms, M.cnt, sg, w.z, 0.0f, ly, ls, la, lt);
+    mfma16_block<false, NACC, 1, RS>(ms, M.cnt, sg, w.z, 0.0f, ly, ls, la,
+                                     lt);
   }
 }
 
@@ -246,6 +247,12 @@ const EnsVariant kEnsMixed[] = {
      ens_mfma16_kernel<5>},        // 8: register budget for 5 waves per SIMD
     {nullptr, nullptr, 256, 1, "persistent_mfma16_occ5_acc2_block256_near3",
      nullptr, ens_mfma16_kernel<5, 2>},  // 9: two weight-product accumulators
+    // 10-11: one reciprocal per four distances
+    {nullptr, nullptr, 256, 1, "persistent_mfma16_occ5_qrcp_block256_near3",
+     nullptr, ens_mfma16_kernel<5, 1, 4>},
+    {nullptr, nullptr, 256, 1,
+     "persistent_mfma16_occ5_acc2_qrcp_block256_near3", nullptr,
+     ens_mfma16_kernel<5, 2, 4>},
 };
 constexpr int kNumEnsF64 = sizeof(kEnsF64) / sizeof(kEnsF64[0]);
 constexpr int kNumEnsMixed = sizeof(kEnsMixed) / sizeof(kEnsMixed[0]);

@@ -216,8 +216,9 @@ __global__ __launch_bounds__(256) void bve_vel_mfma_kernel(
   mfma_block<EPS>(src, *count, sg, blk, eps2x2, ly, lq, ls, lt);
 }
 
-// MINW: minimum waves per SIMD asked of the register allocator (1: none).
-template <bool EPS, int MINW = 1, int NACC = 1>
+// MINW: minimum waves per SIMD asked of the register allocator (1: none);
+// RS: distances per reciprocal (1, 2 or 4).
+template <bool EPS, int MINW = 1, int NACC = 1, int RS = 1>
 __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
     Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
     float eps2x2) {
@@ -228,13 +229,14 @@ __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_kernel(
   const bool first = static_cast<int>(blockIdx.x) < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? blockIdx.x : blockIdx.x - s0.nblocks;
-  mfma16_block<EPS, NACC>(src, *count, sg, blk, eps2x2, ly, ls, la, lt);
+  mfma16_block<EPS, NACC, 1, RS>(src, *count, sg, blk, eps2x2, ly, ls, la,
+                                 lt);
 }
 
 // Source-split form of bve_vel_mfma16_kernel (block b * KS + k: chunk k of
 // target block b, partial sums to sg.part; bve_split_reduce_kernel adds
 // them): enough equal work items that the last round of waves is full.
-template <bool EPS, int MINW, int NACC, int KS>
+template <bool EPS, int MINW, int NACC, int KS, int RS = 1>
 __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_split_kernel(
     Mfma16Src src, const int32_t* __restrict__ count, SegArgs s0, SegArgs s1,
     float eps2x2) {
@@ -247,8 +249,8 @@ __global__ __launch_bounds__(256, MINW) void bve_vel_mfma16_split_kernel(
   const bool first = b < s0.nblocks;
   const SegArgs& sg = pick(s0, s1, first);
   const int blk = first ? b : b - s0.nblocks;
-  mfma16_block<EPS, NACC, KS>(src, *count, sg, blk, eps2x2, ly, ls, la, lt,
-                              chunk);
+  mfma16_block<EPS, NACC, KS, RS>(src, *count, sg, blk, eps2x2, ly, ls, la,
+                                  lt, chunk);
 }
 
 // ------------------------------------------------------ stream function ---
@@ -496,6 +498,28 @@ const MixedVariant kMixedVariants[] = {
      "mfma16_occ5_acc2_block256_near3_split8",
      bve_vel_mfma16_split_kernel<false, 5, 2, 8>,
      bve_vel_mfma16_split_kernel<true, 5, 2, 8>, 8, 10},                // 13
+    // 14-18: variants 10 (and its 8-chunk split) and 9 with one reciprocal
+    // per two (prcp) or four (qrcp) distances
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_prcp_block256_near3",
+     bve_vel_mfma16_kernel<false, 5, 2, 2>,
+     bve_vel_mfma16_kernel<true, 5, 2, 2>},                              // 14
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_prcp_block256_near3_split8",
+     bve_vel_mfma16_split_kernel<false, 5, 2, 8, 2>,
+     bve_vel_mfma16_split_kernel<true, 5, 2, 8, 2>, 8, 14},             // 15
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_qrcp_block256_near3",
+     bve_vel_mfma16_kernel<false, 5, 2, 4>,
+     bve_vel_mfma16_kernel<true, 5, 2, 4>},                              // 16
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_acc2_qrcp_block256_near3_split8",
+     bve_vel_mfma16_split_kernel<false, 5, 2, 8, 4>,
+     bve_vel_mfma16_split_kernel<true, 5, 2, 8, 4>, 8, 16},             // 17
+    {nullptr, nullptr, nullptr, nullptr, 256, 1,
+     "mfma16_occ5_qrcp_block256_near3",
+     bve_vel_mfma16_kernel<false, 5, 1, 4>,
+     bve_vel_mfma16_kernel<true, 5, 1, 4>},                              // 18
 };
 constexpr int kNumF64 = sizeof(kF64Variants) / sizeof(kF64Variants[0]);
 constexpr int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
