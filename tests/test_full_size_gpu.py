@@ -130,3 +130,52 @@ def test_swe_sums_at_benchmark_size(ctx, ora):
     assert rel_l2(d64[sv], rdd) < TOL64
     assert rel_l2(l64[sv], rlap) < TOL64
     assert _tangent(u64, m.vert_x) < TOL64
+
+
+@pytest.mark.parametrize("seed,depth,stride", [(0, 7, 1024), (1, 8, 2048)])
+def test_autotuned_variants_at_benchmark_size(ctx, ora, seed, depth, stride):
+    """The variants the autotuner picks at these sizes, through
+    init_velocity (where the source-split forms run): the FP64 sum in 32
+    source chunks per target block, and the matrix-core mixed sum in 8
+    chunks with one reciprocal per four distances.  FP64 against the oracle
+    on a sample of vertices and against the fixed unsplit FP64 variant,
+    mixed against FP64."""
+    from lpm_amd.bve import BVESphere
+    k64 = _abi.LPM_KERNEL_BVE_VEL_F64
+    kmx = _abi.LPM_KERNEL_BVE_VEL_MIXED
+    n64, nmx = ctx.variants(k64), ctx.variants(kmx)
+    if seed == 0:
+        zfn = lambda x: G.rh54_vorticity(x, 0.1, -0.1)  # noqa: E731
+    else:
+        zfn = lambda x: G.gaussian_vortex_vorticity(  # noqa: E731
+            x, np.array([0.0, 0.0, 1.0]), 4 * np.pi, 4.0)
+    s = BVESphere(seed, depth, ctx=ctx)
+    s.init_vorticity(zfn)
+    leaf = s.mesh.face_mask == 0
+    out = {}
+    try:
+        for name, kid, v, prec in (
+                ("fixed", k64, ctx.L.lpm_variant_default(k64), "fp64"),
+                ("split", k64, n64.index(
+                    "block256_tpl4_nr1_quad_fused_sy_u2_tile512_split32"), "fp64"),
+                ("mixed", kmx, nmx.index(
+                    "mfma16_occ5_acc2_qrcp_block256_near3_split8"), "mixed")):
+            ctx.set_variant(kid, v)
+            s.init_velocity(prec)
+            torch.cuda.synchronize()
+            out[name] = (s.velocity_verts.cpu().numpy().copy(),
+                         s.velocity_faces.cpu().numpy()[:, leaf].copy())
+    finally:
+        ctx.set_variant(k64, -1)
+        ctx.set_variant(kmx, -1)
+    for a, b in zip(out["split"], out["fixed"]):
+        assert rel_l2(a, b) < 1e-13
+    for a, b in zip(out["mixed"], out["split"]):
+        assert rel_l2(a, b) < TOL_MIXED
+    m = s.mesh
+    zf = s.rel_vort_faces.cpu().numpy()
+    sv = slice(0, m.n_verts, stride)
+    ref = ora.bve_velocity(aos(m.vert_x[:, sv]), aos(m.face_x), zf,
+                           m.face_area, m.face_mask, nthreads=16)
+    assert rel_l2(out["split"][0][:, sv].T, ref) < TOL64
+    assert rel_linf(out["split"][0][:, sv].T, ref) < TOL64
