@@ -179,3 +179,35 @@ def test_autotuned_variants_at_benchmark_size(ctx, ora, seed, depth, stride):
                            m.face_area, m.face_mask, nthreads=16)
     assert rel_l2(out["split"][0][:, sv].T, ref) < TOL64
     assert rel_linf(out["split"][0][:, sv].T, ref) < TOL64
+
+
+def test_stream_function_at_benchmark_size(ctx, ora):
+    """The level-7 stream function, whose launches split every target
+    block's sources into chunks (26 for the 641 vertex blocks, 10 for the
+    1,707 face blocks): against the oracle on a sample of vertex and leaf
+    face targets (1e-12) and against the analytic RH 5-4 stream function
+    (discretisation error)."""
+    m = mesh("icos", 7)
+    z = G.rh54_vorticity(m.face_x, 0.1, -0.1)
+    fx, vx = dev(m.face_x), dev(m.vert_x)
+    args = (dev(z), dev(m.face_area), dev(m.face_mask, torch.uint8))
+    skip = _abi.LPM_FLAG_SKIP_MASKED_TARGETS
+    pv = ctx.bve_stream_fn(vx, fx, *args).cpu().numpy()
+    pf = ctx.bve_stream_fn(fx, fx, *args, collocated=True,
+                           flags=skip).cpu().numpy()
+    sv = slice(0, m.n_verts, 1024)
+    ref = ora.bve_stream_fn(aos(m.vert_x[:, sv]), aos(m.face_x), z,
+                            m.face_area, m.face_mask, nthreads=16)
+    assert rel_l2(pv[sv], ref) < TOL64
+    assert rel_linf(pv[sv], ref) < TOL64
+    leaf = m.face_mask == 0
+    fl = np.flatnonzero(leaf)[::1024]
+    tmask = np.ones(m.n_faces, np.uint8)
+    tmask[fl] = 0
+    ref = ora.bve_stream_fn(aos(m.face_x), aos(m.face_x), z, m.face_area,
+                            m.face_mask, collocated=True, tgt_mask=tmask,
+                            nthreads=16)
+    assert rel_l2(pf[fl], ref[fl]) < TOL64
+    assert rel_linf(pf[fl], ref[fl]) < TOL64
+    # the analytic stream function of the RH 5-4 state (2.4e-5 measured)
+    assert rel_l2(pv, G.rh54_stream_function(m.vert_x, 0.1, -0.1)) < 1e-4
